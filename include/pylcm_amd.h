@@ -1,0 +1,156 @@
+/*
+ * pylcm_amd — C ABI of the MI355X (gfx950) backward-induction kernels.
+ *
+ * The reference has no native entry points on this path: its solve calls
+ * XLA-compiled cores through `PeriodKernel.__call__`
+ * (src/_lcm/solution/contract.py:347-426), once per (regime, period), from the
+ * host loop in src/_lcm/solution/backward_induction.py:155-275 via
+ * `_run_period_kernel` (:327-388). Each function below replaces one piece of
+ * that contract; the comment on each names the reference code it stands in for.
+ * Python binds this header with ctypes (pylcm_amd/_native.py).
+ *
+ * Conventions: every function returns 0 on success or a negative LCM_E* code;
+ * lcm_last_error() returns the message of the calling thread's last failure.
+ * Device pointers are plain `void*` obtained from lcm_alloc. A context owns one
+ * HIP stream; contexts may be used from different threads, one thread each.
+ */
+#ifndef PYLCM_AMD_H
+#define PYLCM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LCM_OK 0
+#define LCM_E_ARG (-1)      /* invalid argument (shape, size, null pointer) */
+#define LCM_E_OOM (-2)      /* device allocation failed */
+#define LCM_E_LAUNCH (-3)   /* kernel launch or HIP runtime failure */
+#define LCM_E_DEVICE (-4)   /* no usable device / init failed */
+
+#define LCM_DTYPE_F32 0
+#define LCM_DTYPE_F64 1
+
+#define LCM_GRID_LIN 0
+#define LCM_GRID_LOG 1
+#define LCM_GRID_IRREG 2
+
+#define LCM_MAX_TARGETS 4
+#define LCM_MAX_CONSTR 4
+#define LCM_MAX_SCALAR 4
+#define LCM_MAX_TASTE_D 8
+
+/*
+ * One (problem, regime, period) of the grid-search solve. Replaces the
+ * compiled `max_Q_over_a` core (src/_lcm/regime_building/max_Q_over_a.py:30-191)
+ * built from `Q_and_F` (src/_lcm/regime_building/Q_and_F.py:58-170).
+ *
+ * Tables hold element (sd, w, d, c) at  sd*X_ssd + w*X_sw + d*n_c + c  where sd
+ * is the flat discrete-state index, w the continuous-state index, d the flat
+ * discrete-action index and c the continuous-action index; a stride of 0
+ * broadcasts. All float pointers are of the model dtype.
+ *   Q(sd,w,d,c) = U + beta * CE,  CE = sum_s ms_s[sd,d]*Vs_s
+ *                                   + interp(EV[sd,d], A*w_pts[w] + B)
+ *   EV[sd,d][k] = sum_r sum_j M_r[(sd*n_d + d)*n_sd_next_r + j] * Vn_r[j*g_n + k]
+ *   feasible iff  Fa_k*w + Fb_k >= 0  (> 0 when bit k of strict_mask is set)
+ * Outputs: V_out[sd*n_w + w] and pol_out[sd*n_w + w] (flat action index
+ * d*n_c + c, first maximiser), or with taste shocks pol_out[(sd*n_w + w)*n_d + d]
+ * (the continuous argmax per discrete action) and V the EV1 logsum.
+ */
+typedef struct LcmGsDesc {
+  void* V_out;
+  int32_t* pol_out;
+  const void* w_pts;
+  const void* U;
+  const void* A;
+  const void* B;
+  const void* Fa[LCM_MAX_CONSTR];
+  const void* Fb[LCM_MAX_CONSTR];
+  const void* Vn[LCM_MAX_TARGETS];
+  const void* M[LCM_MAX_TARGETS];
+  const void* Vs[LCM_MAX_SCALAR];
+  const void* ms[LCM_MAX_SCALAR];
+  const void* g_pts;
+  int64_t U_ssd, U_sw, A_ssd, A_sw, B_ssd, B_sw;
+  int64_t Fa_ssd[LCM_MAX_CONSTR], Fa_sw[LCM_MAX_CONSTR];
+  int64_t Fb_ssd[LCM_MAX_CONSTR], Fb_sw[LCM_MAX_CONSTR];
+  int32_t n_sd_next[LCM_MAX_TARGETS];
+  int32_t n_sd, n_w, n_d, n_c;
+  int32_t n_constr, strict_mask, n_targets, n_scalar;
+  int32_t terminal, g_kind, g_n, has_taste;
+  double beta, taste_scale, g_start, g_stop;
+} LcmGsDesc;
+
+/* A work item: states [w0, w0 + tile) of discrete state sd of descriptor desc. */
+typedef struct LcmGsTask {
+  int32_t desc;
+  int32_t sd;
+  int32_t w0;
+  int32_t pad;
+} LcmGsTask;
+
+typedef struct lcm_ctx lcm_ctx;
+
+/* Select and initialise the device. Replaces JAX backend initialisation. */
+int lcm_init(int device);
+/* Message of this thread's last failure ("" if none). */
+const char* lcm_last_error(void);
+/* Compute units, free and total device memory in bytes. */
+int lcm_device_info(int* n_cu, size_t* free_bytes, size_t* total_bytes);
+
+/* Context = one stream. Replaces XLA's per-device execution stream. */
+int lcm_ctx_create(lcm_ctx** out);
+int lcm_ctx_destroy(lcm_ctx* ctx);
+int lcm_ctx_sync(lcm_ctx* ctx);
+
+/* Device memory. Replaces XLA-owned device buffers (jax.device_put). */
+int lcm_alloc(size_t bytes, void** dptr);
+int lcm_free(void* dptr);
+int lcm_host_alloc(size_t bytes, void** hptr); /* pinned host memory */
+int lcm_host_free(void* hptr);
+int lcm_memcpy_h2d(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lcm_memcpy_d2h(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lcm_memcpy_h2d_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lcm_memcpy_d2h_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lcm_memset_async(lcm_ctx* ctx, void* dst, int value, size_t bytes);
+
+/*
+ * Fused grid-search period: one launch evaluates every task (any mix of
+ * problems, regimes and grid sizes). Replaces `_GridSearchPeriodKernel.__call__`
+ * (src/_lcm/solution/grid_search.py:192-212) for all regimes of a period.
+ * tile = states per workgroup (64, 128 or 256); lds_bytes = dynamic LDS, at
+ * least sizeof(T) * 2 * max(g_n) + 2048.
+ */
+int lcm_gs_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
+                  const LcmGsTask* d_tasks, int n_tasks, int tile,
+                  size_t lds_bytes);
+
+/* HIP graph capture of the work submitted to ctx between begin and end.
+ * Replaces the host loop's per-period dispatch (backward_induction.py:155-275). */
+int lcm_graph_begin(lcm_ctx* ctx);
+int lcm_graph_end(lcm_ctx* ctx, void** graph_exec);
+int lcm_graph_launch(lcm_ctx* ctx, void* graph_exec);
+int lcm_graph_destroy(void* graph_exec);
+
+/* Event timing on the context's stream (milliseconds between two records). */
+int lcm_event_create(void** ev);
+int lcm_event_destroy(void* ev);
+int lcm_event_record(lcm_ctx* ctx, void* ev);
+int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms);
+
+/* sizeof the structs above, so a binding can check its mirror of the layout. */
+int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task);
+
+/* Variant of lcm_gs_launch with an explicit choice of the kernel that reads
+ * tables varying with the continuous state (wdep = 1) or not (wdep = 0). */
+int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
+                     const LcmGsTask* d_tasks, int n_tasks, int tile,
+                     size_t lds_bytes, int wdep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYLCM_AMD_H */

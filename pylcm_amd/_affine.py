@@ -1,0 +1,211 @@
+"""Affine tracer for partial evaluation of user functions.
+
+The grid-search kernel evaluates, for every state-action pair, the next
+continuous state and the constraints. User functions are plain numpy-style
+Python, so instead of compiling them the host calls them once per period with
+every discrete variable and the continuous action as concrete numpy arrays, and
+the continuous state as an `Affine` object `a * w + b`. Arithmetic that stays
+affine in `w` (the budget laws `R * (w - c) + y` and constraints `c <= w` of the
+reference's examples) comes back as coefficient tables `a`, `b` over the
+(discrete state, discrete action, continuous action) grid; anything else raises
+`NonAffine` and the caller tabulates that function over the full state grid
+instead. The kernel then forms `fma(a, w, b)` per state-action pair.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+class NonAffine(Exception):
+    """The traced expression is not affine in the continuous state."""
+
+
+def _is_affine(x):
+    return isinstance(x, Affine)
+
+
+def _parts(x):
+    if isinstance(x, Affine):
+        return x.a, x.b
+    return 0.0, np.asarray(x, dtype=np.float64)
+
+
+class Affine:
+    """`a * w + b` with numpy-array coefficients (broadcastable)."""
+
+    __array_priority__ = 1000
+
+    def __init__(self, a, b):
+        self.a = np.asarray(a, dtype=np.float64)
+        self.b = np.asarray(b, dtype=np.float64)
+
+    # -- helpers -------------------------------------------------------------
+    @property
+    def is_const(self) -> bool:
+        return not np.any(self.a != 0)
+
+    def const_value(self):
+        return self.b
+
+    def _simplify(self):
+        return self.b if self.is_const else self
+
+    # -- arithmetic ----------------------------------------------------------
+    def __add__(self, o):
+        if isinstance(o, Constraint):
+            return NotImplemented
+        oa, ob = _parts(o)
+        return Affine(self.a + oa, self.b + ob)
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        oa, ob = _parts(o)
+        return Affine(self.a - oa, self.b - ob)
+
+    def __rsub__(self, o):
+        oa, ob = _parts(o)
+        return Affine(oa - self.a, ob - self.b)
+
+    def __mul__(self, o):
+        if _is_affine(o):
+            if o.is_const:
+                o = o.b
+            elif self.is_const:
+                return o * self.b
+            else:
+                raise NonAffine("product of two state-dependent terms")
+        k = np.asarray(o, dtype=np.float64)
+        return Affine(self.a * k, self.b * k)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        if _is_affine(o):
+            if not o.is_const:
+                raise NonAffine("division by a state-dependent term")
+            o = o.b
+        k = np.asarray(o, dtype=np.float64)
+        return Affine(self.a / k, self.b / k)
+
+    def __rtruediv__(self, o):
+        if self.is_const:
+            return np.asarray(o, dtype=np.float64) / self.b
+        raise NonAffine("division by a state-dependent term")
+
+    def __neg__(self):
+        return Affine(-self.a, -self.b)
+
+    def __pos__(self):
+        return self
+
+    def __pow__(self, k):
+        if self.is_const:
+            return self.b**k
+        if np.isscalar(k) and k == 1:
+            return self
+        raise NonAffine("power of a state-dependent term")
+
+    def __rpow__(self, base):
+        if self.is_const:
+            return np.asarray(base, dtype=np.float64) ** self.b
+        raise NonAffine("state-dependent exponent")
+
+    # -- comparisons give constraints ---------------------------------------
+    def __le__(self, o):
+        return Constraint.make(_as_aff(o) - self, strict=False)
+
+    def __lt__(self, o):
+        return Constraint.make(_as_aff(o) - self, strict=True)
+
+    def __ge__(self, o):
+        return Constraint.make(self - o, strict=False)
+
+    def __gt__(self, o):
+        return Constraint.make(self - o, strict=True)
+
+    def __bool__(self):
+        raise NonAffine("truth value of a state-dependent expression")
+
+    # -- numpy protocol ------------------------------------------------------
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        if method != "__call__" or kwargs.get("out") is not None:
+            raise NonAffine(f"unsupported ufunc use {ufunc.__name__}.{method}")
+        table = {
+            np.add: lambda x, y: _as_aff(x) + y,
+            np.subtract: lambda x, y: _as_aff(x) - y,
+            np.multiply: lambda x, y: _as_aff(x) * y,
+            np.true_divide: lambda x, y: _as_aff(x) / y,
+            np.negative: lambda x: -_as_aff(x),
+            np.positive: lambda x: x,
+            np.less_equal: lambda x, y: _as_aff(x) <= y,
+            np.less: lambda x, y: _as_aff(x) < y,
+            np.greater_equal: lambda x, y: _as_aff(x) >= y,
+            np.greater: lambda x, y: _as_aff(x) > y,
+        }
+        if ufunc in table:
+            return table[ufunc](*inputs)
+        if all((not _is_affine(i)) or i.is_const for i in inputs):
+            args = [i.b if _is_affine(i) else i for i in inputs]
+            return ufunc(*args, **kwargs)
+        raise NonAffine(f"{ufunc.__name__} of a state-dependent term")
+
+    def __array_function__(self, func, types, args, kwargs):
+        if func is np.where and len(args) == 3 and not kwargs:
+            cond, x, y = args
+            if isinstance(cond, (Affine, Constraint)):
+                raise NonAffine("condition depends on the continuous state")
+            xa, xb = _parts(x)
+            ya, yb = _parts(y)
+            return Affine(np.where(cond, xa, ya), np.where(cond, xb, yb))._simplify()
+        flat = list(args) + list(kwargs.values())
+        if all((not _is_affine(i)) or i.is_const for i in flat):
+            args2 = [i.b if _is_affine(i) else i for i in args]
+            kw2 = {k: (v.b if _is_affine(v) else v) for k, v in kwargs.items()}
+            return func(*args2, **kw2)
+        raise NonAffine(f"{func.__name__} of a state-dependent term")
+
+
+def _as_aff(x):
+    return x if isinstance(x, Affine) else Affine(0.0, x)
+
+
+class Constraint:
+    """Conjunction of affine inequalities `a_k * w + b_k >= 0` (or `> 0`)."""
+
+    def __init__(self, terms):
+        self.terms = list(terms)  # [(a, b, strict)] ; a concrete bool mask is (0, 0|-inf, False)
+
+    @staticmethod
+    def make(expr, strict):
+        expr = _as_aff(expr)
+        if expr.is_const:
+            return (expr.b > 0) if strict else (expr.b >= 0)
+        return Constraint([(expr.a, expr.b, strict)])
+
+    def __and__(self, o):
+        return Constraint(self.terms + _constraint_terms(o))
+
+    __rand__ = __and__
+
+    def __bool__(self):
+        raise NonAffine("truth value of a state-dependent constraint")
+
+    def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
+        if ufunc in (np.logical_and, np.bitwise_and) and method == "__call__":
+            terms = []
+            for i in inputs:
+                terms += _constraint_terms(i)
+            return Constraint(terms)
+        raise NonAffine(f"{ufunc.__name__} of a state-dependent constraint")
+
+    def __array_function__(self, func, types, args, kwargs):
+        raise NonAffine(f"{func.__name__} of a state-dependent constraint")
+
+
+def _constraint_terms(x):
+    if isinstance(x, Constraint):
+        return x.terms
+    mask = np.asarray(x, dtype=bool)
+    return [(np.zeros(()), np.where(mask, 0.0, -np.inf), False)]

@@ -1,0 +1,591 @@
+"""Regime layouts and per-period lowering of a model into kernel tables.
+
+`RegimeLayout` fixes the array layout the reference uses for a regime
+(`src/_lcm/regime_building/V.py:367-388`, `max_Q_over_a.py:66-67`): value
+function axes are the discrete and process states in declaration order
+followed by the one continuous state; the policy index is the flat index over
+the discrete actions (declaration order) followed by the continuous action.
+
+`lower_period` turns one (regime, period) of a grid-search regime into the
+tables the fused HIP kernel consumes:
+
+* `U[sd, w, d, c]` utility (`w` extent 1 unless utility reads the continuous
+  state), from `_get_U_and_F` (`Q_and_F.py:1321-1351`);
+* `A, B` with `w' = A * w + B` for the continuous state's law of motion
+  (`next_state.py`), found by the affine tracer;
+* constraint rows `(Fa, Fb, strict)`: feasible iff `Fa * w + Fb >= 0` (`> 0`)
+  (`Q_and_F.py:1354-1384`);
+* per stateful target regime a weight block `M[sd, d, sd']` that folds the
+  regime transition probability (`compute_CE`, `Q_and_F.py:466-681`), the joint
+  next-node weights normalised by their sum (`_expectation_over_stochastic_nodes`,
+  `:1145-1209`) and the division by the represented regime mass (`:639-653`);
+  and per stateless target a weight `m[sd, d]`.
+
+Because interpolation is linear, `sum_k w_k interp(V_k, w') = interp(sum_k w_k
+V_k, w')`; the kernel therefore contracts V_{t+1} with `M` once per row and
+interpolates the contracted row once per state-action pair. A zero weight
+drops its term whatever value stands there (`zero_safe.py:161-251`).
+"""
+
+from __future__ import annotations
+
+import hashlib
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from pylcm_amd._affine import Affine, Constraint, NonAffine
+from pylcm_amd._dag import Evaluator, unwrap
+from pylcm_amd.exceptions import (
+    InvalidParamsError,
+    ModelInitializationError,
+    UnsupportedModelError,
+)
+from pylcm_amd.grids import ContinuousGrid, DiscreteGrid, GRID_KIND_IRREG
+from pylcm_amd.processes import StochasticProcess
+from pylcm_amd.regime import MarkovTransition
+from pylcm_amd.solvers import DCEGM, GridSearch
+
+MAX_REGIME_MASS_DEVIATION = 1.0e-3
+
+
+@dataclass
+class RegimeLayout:
+    name: str
+    code: int
+    regime: Any
+    sd_names: tuple
+    sd_grids: tuple
+    w_name: str | None
+    w_grid: Any
+    da_names: tuple
+    da_grids: tuple
+    c_name: str | None
+    c_grid: Any
+
+    @property
+    def sd_sizes(self):
+        return tuple(g.n_points for g in self.sd_grids)
+
+    @property
+    def da_sizes(self):
+        return tuple(g.n_points for g in self.da_grids)
+
+    @property
+    def n_sd(self):
+        return int(np.prod(self.sd_sizes, dtype=np.int64)) if self.sd_sizes else 1
+
+    @property
+    def n_w(self):
+        return self.w_grid.n_points if self.w_grid is not None else 1
+
+    @property
+    def n_d(self):
+        return int(np.prod(self.da_sizes, dtype=np.int64)) if self.da_sizes else 1
+
+    @property
+    def n_c(self):
+        return self.c_grid.n_points if self.c_grid is not None else 1
+
+    @property
+    def V_shape(self):
+        return self.sd_sizes + ((self.n_w,) if self.w_grid is not None else ())
+
+    @property
+    def terminal(self):
+        return self.regime.transition is None
+
+    @property
+    def state_names(self):
+        return self.sd_names + ((self.w_name,) if self.w_name else ())
+
+    @property
+    def action_names(self):
+        return self.da_names + ((self.c_name,) if self.c_name else ())
+
+    @property
+    def is_dcegm(self):
+        return isinstance(self.regime.solver, DCEGM)
+
+
+def build_layout(name, code, regime) -> RegimeLayout:
+    sd_n, sd_g, cont = [], [], []
+    for s, g in regime.states.items():
+        if g is None:
+            continue
+        if isinstance(g, (DiscreteGrid, StochasticProcess)):
+            sd_n.append(s)
+            sd_g.append(g)
+        elif isinstance(g, ContinuousGrid):
+            cont.append((s, g))
+        else:
+            raise UnsupportedModelError(f"regime {name!r}: unsupported grid for state {s!r}")
+    if len(cont) > 1:
+        raise UnsupportedModelError(
+            f"regime {name!r}: the HIP kernels interpolate one continuous state; got {len(cont)}"
+        )
+    da_n, da_g, cact = [], [], []
+    for a, g in regime.actions.items():
+        if g is None:
+            continue
+        if isinstance(g, DiscreteGrid):
+            da_n.append(a)
+            da_g.append(g)
+        elif isinstance(g, ContinuousGrid) and not isinstance(g, StochasticProcess):
+            cact.append((a, g))
+        else:
+            raise UnsupportedModelError(f"regime {name!r}: unsupported grid for action {a!r}")
+    if len(cact) > 1:
+        raise UnsupportedModelError(
+            f"regime {name!r}: the HIP kernels search one continuous action; got {len(cact)}"
+        )
+    if not isinstance(regime.solver, (GridSearch, DCEGM)):
+        raise UnsupportedModelError(f"regime {name!r}: solver {type(regime.solver).__name__} is not implemented")
+    return RegimeLayout(
+        name=name, code=code, regime=regime,
+        sd_names=tuple(sd_n), sd_grids=tuple(sd_g),
+        w_name=cont[0][0] if cont else None, w_grid=cont[0][1] if cont else None,
+        da_names=tuple(da_n), da_grids=tuple(da_g),
+        c_name=cact[0][0] if cact else None, c_grid=cact[0][1] if cact else None,
+    )
+
+
+# ---------------------------------------------------------------------------
+# per-solve context: grids resolved with runtime params, in the model dtype
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class ResolvedGrids:
+    """Grid points of every regime for one params set (model dtype values)."""
+
+    dtype: np.dtype
+    sd_values: dict = field(default_factory=dict)   # regime -> [values per sd axis] (f64)
+    proc: dict = field(default_factory=dict)        # (regime, state) -> (nodes f64, P f64)
+    w_pts: dict = field(default_factory=dict)       # regime -> f64 points (dtype-rounded)
+    c_pts: dict = field(default_factory=dict)
+
+
+def process_runtime_params(params, fixed, regime_name, state_name):
+    out = {}
+    for src in (fixed or {}, params or {}):
+        r = src.get(regime_name)
+        if isinstance(r, dict) and isinstance(r.get(state_name), dict):
+            out.update(r[state_name])
+        if isinstance(src.get(state_name), dict):
+            for k, v in src[state_name].items():
+                out.setdefault(k, v)
+    return out
+
+
+def resolve_grids(layouts, params, fixed, dtype) -> ResolvedGrids:
+    dt = np.dtype(dtype)
+    rg = ResolvedGrids(dtype=dt)
+    for lay in layouts.values():
+        vals = []
+        for s, g in zip(lay.sd_names, lay.sd_grids):
+            if isinstance(g, StochasticProcess):
+                nodes, P = g.nodes_and_matrix(process_runtime_params(params, fixed, lay.name, s))
+                nodes = nodes.astype(dt).astype(np.float64)
+                rg.proc[(lay.name, s)] = (nodes, np.asarray(P, dtype=np.float64))
+                vals.append(nodes)
+            else:
+                vals.append(np.arange(g.n_points, dtype=np.float64))
+        rg.sd_values[lay.name] = vals
+        if lay.w_grid is not None:
+            rg.w_pts[lay.name] = lay.w_grid.to_numpy(dt).astype(np.float64)
+        if lay.c_grid is not None:
+            rg.c_pts[lay.name] = lay.c_grid.to_numpy(dt).astype(np.float64)
+    return rg
+
+
+# ---------------------------------------------------------------------------
+# lowering
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class PeriodTables:
+    regime: str
+    period: int
+    terminal: bool
+    U: np.ndarray
+    A: np.ndarray | None = None
+    B: np.ndarray | None = None
+    F: list = field(default_factory=list)
+    targets: list = field(default_factory=list)     # [(name, M[n_sd, n_d, n_sd_r])]
+    scalars: list = field(default_factory=list)     # [(name, m[n_sd, n_d])]
+    beta: float = 0.0
+    taste_scale: float | None = None
+    next_grid: Any = None                             # target regime name sharing the grid
+
+
+def _compress(arr, n_sd, n_w, n_d, n_c):
+    """Broadcast to (n_sd, n_w, n_d, n_c); collapse sd / w axes that do not vary."""
+    full = np.broadcast_to(arr, (n_sd, n_w, n_d, n_c))
+    if n_w > 1 and np.all(full == full[:, :1]) :
+        full = full[:, :1]
+    if n_sd > 1 and np.all(full == full[:1]):
+        full = full[:1]
+    return np.ascontiguousarray(full, dtype=np.float64)
+
+
+class _Axes:
+    """Broadcast axes [sd..., w, da..., c] of one regime's evaluation grid."""
+
+    def __init__(self, lay: RegimeLayout, rg: ResolvedGrids, *, full_w: bool):
+        self.lay = lay
+        n_sdax = len(lay.sd_names)
+        n_daax = len(lay.da_names)
+        self.ndim = n_sdax + 1 + n_daax + 1
+        self.w_axis = n_sdax
+        self.vars = {}
+
+        def on_axis(v, ax):
+            shape = [1] * self.ndim
+            shape[ax] = len(v)
+            return np.asarray(v).reshape(shape)
+
+        for i, (s, vals) in enumerate(zip(lay.sd_names, rg.sd_values[lay.name])):
+            g = lay.sd_grids[i]
+            v = vals.astype(np.int64) if isinstance(g, DiscreteGrid) else vals
+            self.vars[s] = on_axis(v, i)
+        for j, a in enumerate(lay.da_names):
+            self.vars[a] = on_axis(np.arange(lay.da_grids[j].n_points, dtype=np.int64), n_sdax + 1 + j)
+        if lay.c_name:
+            self.vars[lay.c_name] = on_axis(rg.c_pts[lay.name], self.ndim - 1)
+        if lay.w_name:
+            if full_w:
+                self.vars[lay.w_name] = on_axis(rg.w_pts[lay.name], self.w_axis)
+            else:
+                self.vars[lay.w_name] = Affine(1.0, 0.0)
+        self.full_w = full_w
+
+    def to4(self, arr):
+        """Reshape a broadcast result on [sd..., w, da..., c] to (n_sd, n_w, n_d, n_c)."""
+        lay = self.lay
+        n_w = lay.n_w if self.full_w else 1
+        shape = lay.sd_sizes + (n_w,) + lay.da_sizes + (lay.n_c,)
+        a = np.broadcast_to(np.asarray(arr, dtype=np.float64), shape)
+        return a.reshape(lay.n_sd, n_w, lay.n_d, lay.n_c)
+
+
+def _functions_of(lay: RegimeLayout):
+    funcs = {k: v for k, v in lay.regime.functions.items() if v is not None}
+    return funcs
+
+
+class PeriodLowering:
+    """Lowers every regime of a model at one period for one params set."""
+
+    def __init__(self, model, params, rg: ResolvedGrids):
+        self.model = model
+        self.params = params
+        self.fixed = model.fixed_params
+        self.rg = rg
+
+    def _evaluator(self, lay, variables, period):
+        age = self.model.ages.period_to_age(period)
+        v = dict(variables)
+        v["age"] = age
+        v["period"] = period
+        return Evaluator(regime_name=lay.name, functions=_functions_of(lay), variables=v,
+                         params=self.params, fixed_params=self.fixed, param_dtype=self.rg.dtype)
+
+    # -- scalar-valued functions on the broadcast grid ------------------------
+    def _tabulate(self, lay, period, fname, func=None):
+        """(result on the symbolic grid | None, full-w table | None)."""
+        ax = _Axes(lay, self.rg, full_w=False)
+        ev = self._evaluator(lay, ax.vars, period)
+        try:
+            out = ev.call(fname, func)
+            if isinstance(out, Affine):
+                return ("affine", out, ax)
+            if isinstance(out, Constraint):
+                return ("constraint", out, ax)
+            return ("const", out, ax)
+        except NonAffine:
+            axf = _Axes(lay, self.rg, full_w=True)
+            evf = self._evaluator(lay, axf.vars, period)
+            return ("full", evf.call(fname, func), axf)
+
+    def utility(self, lay, period):
+        kind, out, ax = self._tabulate(lay, period, "utility")
+        if kind == "affine":
+            if not out.is_const:
+                ax = _Axes(lay, self.rg, full_w=True)
+                out = self._evaluator(lay, ax.vars, period).call("utility")
+            else:
+                out = out.b
+        return _compress(ax.to4(out), lay.n_sd, ax.to4(out).shape[1], lay.n_d, lay.n_c)
+
+    def constraints(self, lay, period):
+        rows = []
+        for cname, cfun in lay.regime.constraints.items():
+            if cfun is None:
+                continue
+            kind, out, ax = self._tabulate(lay, period, cname, cfun)
+            if kind == "constraint":
+                for a, b, strict in out.terms:
+                    A4 = ax.to4(a)
+                    B4 = ax.to4(b)
+                    rows.append((_compress(A4, lay.n_sd, 1, lay.n_d, lay.n_c),
+                                 _compress(B4, lay.n_sd, 1, lay.n_d, lay.n_c), bool(strict)))
+            else:
+                if kind == "affine":
+                    raise UnsupportedModelError(f"constraint {cname!r} does not return a boolean")
+                mask = ax.to4(np.asarray(out, dtype=bool))
+                if np.all(mask):
+                    continue
+                fb = np.where(mask, 0.0, -np.inf)
+                rows.append((_compress(np.zeros(()), lay.n_sd, 1, lay.n_d, lay.n_c),
+                             _compress(fb, lay.n_sd, mask.shape[1], lay.n_d, lay.n_c), False))
+        return rows
+
+    def next_continuous(self, lay, period):
+        law = lay.regime.state_transitions.get(lay.w_name)
+        if law is None:
+            raise ModelInitializationError(f"regime {lay.name!r}: no transition for state {lay.w_name!r}")
+        if isinstance(law, dict) or hasattr(law, "items"):
+            raise UnsupportedModelError(f"regime {lay.name!r}: per-target laws for {lay.w_name!r} are not implemented")
+        if isinstance(law, MarkovTransition):
+            raise UnsupportedModelError("a continuous state cannot have a Markov transition")
+        fname = f"next_{lay.w_name}"
+        kind, out, ax = self._tabulate(lay, period, fname, law)
+        if kind == "affine":
+            A4, B4 = ax.to4(out.a), ax.to4(out.b)
+            return (_compress(A4, lay.n_sd, 1, lay.n_d, lay.n_c),
+                    _compress(B4, lay.n_sd, 1, lay.n_d, lay.n_c))
+        if kind == "constraint":
+            raise UnsupportedModelError(f"{fname} returns a boolean")
+        t = ax.to4(out)
+        return (_compress(np.zeros(()), lay.n_sd, 1, lay.n_d, lay.n_c),
+                _compress(t, lay.n_sd, t.shape[1], lay.n_d, lay.n_c))
+
+    # -- pointwise laws over (sd, da) -----------------------------------------
+    def _points(self, lay):
+        sdv = self.rg.sd_values[lay.name]
+        sd_index = np.indices(lay.sd_sizes).reshape(len(lay.sd_sizes), -1).T if lay.sd_sizes else np.zeros((1, 0), int)
+        da_index = np.indices(lay.da_sizes).reshape(len(lay.da_sizes), -1).T if lay.da_sizes else np.zeros((1, 0), int)
+        for i, sdi in enumerate(sd_index):
+            for j, dai in enumerate(da_index):
+                v = {}
+                for k, s in enumerate(lay.sd_names):
+                    g = lay.sd_grids[k]
+                    val = sdv[k][sdi[k]]
+                    v[s] = int(val) if isinstance(g, DiscreteGrid) else float(val)
+                for k, a in enumerate(lay.da_names):
+                    v[a] = int(dai[k])
+                if lay.w_name:
+                    v[lay.w_name] = Affine(1.0, 0.0)
+                yield i, j, v
+
+    def _pointwise(self, lay, period, fname, func, variables):
+        ev = self._evaluator(lay, variables, period)
+        if lay.c_name and lay.c_name in _arg_closure(lay, func):
+            raise UnsupportedModelError(
+                f"regime {lay.name!r}: {fname} depends on the continuous action; not implemented"
+            )
+        try:
+            out = ev.call(fname, func)
+        except NonAffine as e:
+            raise UnsupportedModelError(
+                f"regime {lay.name!r}: {fname} depends on the continuous state; not implemented"
+            ) from e
+        if isinstance(out, (Affine, Constraint)):
+            if isinstance(out, Affine) and out.is_const:
+                return out.b
+            raise UnsupportedModelError(
+                f"regime {lay.name!r}: {fname} depends on the continuous state; not implemented"
+            )
+        return out
+
+    def regime_probs(self, lay, period):
+        """p[sd, d, regime_code] of moving to each regime."""
+        n_reg = len(self.model.regime_names)
+        p = np.zeros((lay.n_sd, lay.n_d, n_reg))
+        tr = lay.regime.transition
+        for i, j, v in self._points(lay):
+            if isinstance(tr, dict) or hasattr(tr, "items"):
+                for tname, f in tr.items():
+                    val = self._pointwise(lay, period, "next_regime", f, v)
+                    p[i, j, self.model.regime_code(tname)] = float(np.asarray(val).reshape(()))
+            elif isinstance(tr, MarkovTransition):
+                val = np.asarray(self._pointwise(lay, period, "next_regime", tr, v), dtype=np.float64)
+                if val.shape != (n_reg,):
+                    raise InvalidParamsError(
+                        f"regime {lay.name!r}: Markov regime transition must return {n_reg} probabilities"
+                    )
+                p[i, j, :] = val
+            else:
+                val = int(np.asarray(self._pointwise(lay, period, "next_regime", tr, v)).reshape(()))
+                if not 0 <= val < n_reg:
+                    raise InvalidParamsError(f"regime {lay.name!r}: next regime id {val} out of range")
+                p[i, j, val] = 1.0
+        return p
+
+    def node_weights(self, lay, tgt, period):
+        """W[sd, d, sd_tgt]: joint weights of the target's discrete/process nodes."""
+        per_axis = []
+        for k, (s, g) in enumerate(zip(tgt.sd_names, tgt.sd_grids)):
+            W = np.zeros((lay.n_sd, lay.n_d, g.n_points))
+            if isinstance(g, StochasticProcess):
+                if s not in lay.sd_names:
+                    raise UnsupportedModelError(
+                        f"process state {s!r} of {tgt.name!r} is not a state of {lay.name!r}"
+                    )
+                nodes_t, P_t = self.rg.proc[(tgt.name, s)]
+                src_k = lay.sd_names.index(s)
+                for i, j, v in self._points(lay):
+                    W[i, j] = interpolated_row(P_t, nodes_t, v[s])
+            else:
+                law = lay.regime.state_transitions.get(s)
+                if law is None and s in lay.sd_names:
+                    raise ModelInitializationError(f"regime {lay.name!r}: no transition for state {s!r}")
+                if law is None:
+                    raise UnsupportedModelError(f"target {tgt.name!r}: no law for entering state {s!r}")
+                if isinstance(law, dict) or hasattr(law, "items"):
+                    law = law[tgt.name]
+                for i, j, v in self._points(lay):
+                    val = np.asarray(self._pointwise(lay, period, f"next_{s}", law, v), dtype=np.float64)
+                    if isinstance(law, MarkovTransition):
+                        if val.shape != (g.n_points,):
+                            raise InvalidParamsError(
+                                f"Markov transition of {s!r} must return {g.n_points} probabilities"
+                            )
+                        W[i, j] = val
+                    else:
+                        code = int(val.reshape(()))
+                        if not 0 <= code < g.n_points:
+                            raise InvalidParamsError(f"next {s!r} code {code} out of range")
+                        W[i, j, code] = 1.0
+            per_axis.append(W)
+        if not per_axis:
+            return np.ones((lay.n_sd, lay.n_d, 1))
+        joint = per_axis[0]
+        for W in per_axis[1:]:
+            joint = (joint[..., :, None] * W[..., None, :]).reshape(lay.n_sd, lay.n_d, -1)
+        return joint
+
+    def lower(self, lay: RegimeLayout, period: int) -> PeriodTables:
+        model = self.model
+        T = model.n_periods
+        U = self.utility(lay, period)
+        F = self.constraints(lay, period)
+        pt = PeriodTables(regime=lay.name, period=period, terminal=lay.terminal, U=U, F=F)
+        if lay.terminal:
+            return pt
+        if period == T - 1:
+            raise ModelInitializationError(
+                f"non-terminal regime {lay.name!r} is active in the last period"
+            )
+        ev = self._evaluator(lay, {}, period)
+        pt.beta = float(ev.param("koopmans_aggregator", "discount_factor"))
+        if lay.regime.taste_shocks is not None:
+            r = (self.params or {}).get(lay.name, {})
+            ts = r.get("taste_shocks", {}) if isinstance(r, dict) else {}
+            if "scale" not in ts:
+                raise InvalidParamsError(f"regime {lay.name!r}: missing params[...]['taste_shocks']['scale']")
+            pt.taste_scale = float(ts["scale"])
+            if not pt.taste_scale > 0:
+                raise InvalidParamsError("taste shock scale must be positive")
+        age_next = model.ages.period_to_age(period + 1)
+        p = self.regime_probs(lay, period)
+        active_next = [r for r in model.regime_names if model.layouts[r].regime.active(age_next)]
+        for r in model.regime_names:
+            code = model.regime_code(r)
+            if r not in active_next and np.any(p[:, :, code] != 0):
+                raise InvalidParamsError(
+                    f"regime {lay.name!r} at period {period} moves to {r!r}, which is inactive next period"
+                )
+        mass = sum(p[:, :, model.regime_code(r)] for r in active_next)
+        has_neg = np.zeros_like(mass, dtype=bool)
+        for r in active_next:
+            has_neg |= np.signbit(p[:, :, model.regime_code(r)]) & (p[:, :, model.regime_code(r)] != 0)
+        ok = (np.abs(mass - 1.0) <= MAX_REGIME_MASS_DEVIATION) & ~has_neg
+        safe_mass = np.where(ok, mass, np.nan)
+        stateful = [r for r in active_next if model.layouts[r].V_shape != ()]
+        scalar = [r for r in active_next if model.layouts[r].V_shape == ()]
+        if stateful:
+            if lay.w_name is None:
+                raise UnsupportedModelError(f"regime {lay.name!r} has no continuous state to hand over")
+            ref = model.layouts[stateful[0]]
+            for r in stateful:
+                tl = model.layouts[r]
+                if tl.w_name != lay.w_name:
+                    raise UnsupportedModelError(
+                        f"target {r!r} has continuous state {tl.w_name!r}, source {lay.w_name!r}"
+                    )
+                if not np.array_equal(self.rg.w_pts[r], self.rg.w_pts[ref.name]) or tl.w_grid.kind != ref.w_grid.kind:
+                    raise UnsupportedModelError("target regimes must share one continuous grid")
+            pt.next_grid = ref.name
+            pt.A, pt.B = self.next_continuous(lay, period)
+        for r in stateful:
+            tl = model.layouts[r]
+            pr = p[:, :, model.regime_code(r)]
+            W = self.node_weights(lay, tl, period)
+            wsum = W.sum(axis=-1, keepdims=True)
+            Wn = W / np.where(wsum > 0, wsum, 1.0)
+            M = np.where(pr[..., None] != 0, pr[..., None] * Wn, 0.0) / safe_mass[..., None]
+            pt.targets.append((r, M))
+        for r in scalar:
+            pr = p[:, :, model.regime_code(r)]
+            pt.scalars.append((r, np.where(pr != 0, pr, 0.0) / safe_mass))
+        if not ok.all():
+            # Poison CE where the regime mass is not a distribution: a NaN weight
+            # on a zero-valued scalar term turns Q into NaN.
+            pt.scalars.append((None, np.where(ok, 0.0, np.nan)))
+        return pt
+
+
+def _arg_closure(lay, func):
+    """Names a function reads, following other regime functions transitively."""
+    from pylcm_amd._dag import arg_names
+
+    funcs = _functions_of(lay)
+    seen, stack, out = set(), [func], set()
+    while stack:
+        f = stack.pop()
+        for a in arg_names(f):
+            out.add(a)
+            if a in funcs and a not in seen:
+                seen.add(a)
+                stack.append(funcs[a])
+    return out
+
+
+def irreg_coordinate(points, value):
+    """Generalised coordinate on an increasing grid (`coordinates.py:117-167`)."""
+    n = len(points)
+    if n == 1:
+        return 0.0
+    hi = int(np.searchsorted(points, value, side="right"))
+    hi = min(max(hi, 1), n - 1)
+    lo = hi - 1
+    return lo + (value - points[lo]) / (points[hi] - points[lo])
+
+
+def interpolated_row(P, nodes, value):
+    """Transition row at an arbitrary current value (`processing.py:2780-2840`)."""
+    n = len(nodes)
+    coord = irreg_coordinate(nodes, value)
+    lo = int(np.clip(np.floor(coord), 0, n - 2)) if n > 1 else 0
+    if n == 1:
+        return P[0].copy()
+    t = coord - lo
+    a = np.where((1 - t) != 0, (1 - t) * P[lo], 0.0)
+    b = np.where(t != 0, t * P[lo + 1], 0.0)
+    return a + b
+
+
+def table_digest(*arrays) -> str:
+    h = hashlib.blake2b(digest_size=16)
+    for a in arrays:
+        if a is None:
+            h.update(b"none")
+            continue
+        a = np.ascontiguousarray(a)
+        h.update(str(a.shape).encode())
+        h.update(a.tobytes())
+    return h.hexdigest()

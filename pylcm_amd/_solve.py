@@ -1,0 +1,45 @@
+"""Host drivers of the GPU solve (plain and batched)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from pylcm_amd._gs_program import GsProgram, lower_problem
+from pylcm_amd.exceptions import UnsupportedModelError
+
+
+def _split(results):
+    V, P = {}, {}
+    for t, regs in results.items():
+        V[t] = {r: v for r, (v, _) in regs.items()}
+        P[t] = {r: p for r, (_, p) in regs.items()}
+    return V, P
+
+
+def _has_dcegm(model):
+    return any(lay.is_dcegm for lay in model.layouts.values())
+
+
+def solve_model(model, params, dtype):
+    """One solve: returns (period->regime->V, period->regime->policy) on the host."""
+    if _has_dcegm(model):
+        from pylcm_amd._egm_program import solve_dcegm_model
+
+        return solve_dcegm_model(model, params, dtype)
+    prog = GsProgram(model, [params], dtype)
+    prog.run(use_graph=False)
+    prog.fetch()
+    return _split(prog.results(0))
+
+
+def solve_batch(model, params_list, dtype, *, tile=None, use_graph=True, return_policy=False):
+    if _has_dcegm(model):
+        raise UnsupportedModelError("batched solves of DC-EGM regimes are not implemented")
+    prog = GsProgram(model, params_list, dtype, tile=tile)
+    prog.run(use_graph=use_graph)
+    prog.fetch()
+    out = []
+    for p in range(len(params_list)):
+        V, P = _split(prog.results(p))
+        out.append((V, P) if return_policy else V)
+    return out

@@ -1,0 +1,219 @@
+// Runtime part of the C ABI: device selection, contexts (one HIP stream each),
+// device and pinned host memory, copies, HIP graph capture and event timing.
+// Declarations and the reference code each entry stands in for are in
+// include/pylcm_amd.h.
+
+#include <cstring>
+#include <string>
+
+#include "common.h"
+
+namespace lcm {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  g_last_error = m;
+  if (e == hipErrorOutOfMemory) return LCM_E_OOM;
+  return LCM_E_LAUNCH;
+}
+
+}  // namespace lcm
+
+using namespace lcm;
+
+extern "C" {
+
+const char* lcm_last_error(void) { return g_last_error.c_str(); }
+
+int lcm_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(LCM_E_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(LCM_E_ARG, "device index out of range");
+  LCM_HIP(hipSetDevice(device));
+  LCM_HIP(hipFree(nullptr));  // force context creation
+  return LCM_OK;
+}
+
+int lcm_device_info(int* n_cu, size_t* free_bytes, size_t* total_bytes) {
+  int dev = 0;
+  LCM_HIP(hipGetDevice(&dev));
+  if (n_cu) {
+    int v = 0;
+    LCM_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+    *n_cu = v;
+  }
+  size_t f = 0, t = 0;
+  LCM_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return LCM_OK;
+}
+
+int lcm_ctx_create(lcm_ctx** out) {
+  if (!out) return fail(LCM_E_ARG, "lcm_ctx_create: null out");
+  lcm_ctx* c = new lcm_ctx();
+  LCM_HIP(hipGetDevice(&c->device));
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return check_hip(e, "hipStreamCreateWithFlags");
+  }
+  *out = c;
+  return LCM_OK;
+}
+
+int lcm_ctx_destroy(lcm_ctx* ctx) {
+  if (!ctx) return LCM_OK;
+  hipError_t e1 = hipStreamSynchronize(ctx->stream);
+  hipError_t e2 = hipStreamDestroy(ctx->stream);
+  delete ctx;
+  if (e1 != hipSuccess) return check_hip(e1, "hipStreamSynchronize");
+  if (e2 != hipSuccess) return check_hip(e2, "hipStreamDestroy");
+  return LCM_OK;
+}
+
+int lcm_ctx_sync(lcm_ctx* ctx) {
+  if (!ctx) return fail(LCM_E_ARG, "lcm_ctx_sync: null ctx");
+  LCM_HIP(hipStreamSynchronize(ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_alloc(size_t bytes, void** dptr) {
+  if (!dptr) return fail(LCM_E_ARG, "lcm_alloc: null out");
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 256;
+  LCM_HIP(hipMalloc(dptr, bytes));
+  return LCM_OK;
+}
+
+int lcm_free(void* dptr) {
+  if (!dptr) return LCM_OK;
+  LCM_HIP(hipFree(dptr));
+  return LCM_OK;
+}
+
+int lcm_host_alloc(size_t bytes, void** hptr) {
+  if (!hptr) return fail(LCM_E_ARG, "lcm_host_alloc: null out");
+  if (bytes == 0) bytes = 256;
+  LCM_HIP(hipHostMalloc(hptr, bytes, hipHostMallocDefault));
+  return LCM_OK;
+}
+
+int lcm_host_free(void* hptr) {
+  if (!hptr) return LCM_OK;
+  LCM_HIP(hipHostFree(hptr));
+  return LCM_OK;
+}
+
+int lcm_memcpy_h2d(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_memcpy_h2d: null argument");
+  if (bytes == 0) return LCM_OK;
+  LCM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  LCM_HIP(hipStreamSynchronize(ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_memcpy_d2h(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_memcpy_d2h: null argument");
+  if (bytes == 0) return LCM_OK;
+  LCM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LCM_HIP(hipStreamSynchronize(ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_memcpy_h2d_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_memcpy_h2d_async: null argument");
+  if (bytes == 0) return LCM_OK;
+  LCM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_memcpy_d2h_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_memcpy_d2h_async: null argument");
+  if (bytes == 0) return LCM_OK;
+  LCM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_memset_async(lcm_ctx* ctx, void* dst, int value, size_t bytes) {
+  if (!ctx || (!dst && bytes)) return fail(LCM_E_ARG, "lcm_memset_async: null argument");
+  if (bytes == 0) return LCM_OK;
+  LCM_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_graph_begin(lcm_ctx* ctx) {
+  if (!ctx) return fail(LCM_E_ARG, "lcm_graph_begin: null ctx");
+  LCM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  return LCM_OK;
+}
+
+int lcm_graph_end(lcm_ctx* ctx, void** graph_exec) {
+  if (!ctx || !graph_exec) return fail(LCM_E_ARG, "lcm_graph_end: null argument");
+  hipGraph_t graph = nullptr;
+  LCM_HIP(hipStreamEndCapture(ctx->stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return check_hip(e, "hipGraphInstantiate");
+  *graph_exec = reinterpret_cast<void*>(exec);
+  return LCM_OK;
+}
+
+int lcm_graph_launch(lcm_ctx* ctx, void* graph_exec) {
+  if (!ctx || !graph_exec) return fail(LCM_E_ARG, "lcm_graph_launch: null argument");
+  LCM_HIP(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return LCM_OK;
+  LCM_HIP(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec)));
+  return LCM_OK;
+}
+
+int lcm_event_create(void** ev) {
+  if (!ev) return fail(LCM_E_ARG, "lcm_event_create: null out");
+  hipEvent_t e;
+  LCM_HIP(hipEventCreate(&e));
+  *ev = reinterpret_cast<void*>(e);
+  return LCM_OK;
+}
+
+int lcm_event_destroy(void* ev) {
+  if (!ev) return LCM_OK;
+  LCM_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)));
+  return LCM_OK;
+}
+
+int lcm_event_record(lcm_ctx* ctx, void* ev) {
+  if (!ctx || !ev) return fail(LCM_E_ARG, "lcm_event_record: null argument");
+  LCM_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), ctx->stream));
+  return LCM_OK;
+}
+
+int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms) {
+  if (!ev_start || !ev_stop || !ms) return fail(LCM_E_ARG, "lcm_event_elapsed: null argument");
+  LCM_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev_stop)));
+  LCM_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(ev_start), reinterpret_cast<hipEvent_t>(ev_stop)));
+  return LCM_OK;
+}
+
+}  // extern "C"
+
+// Sizes of the structs shared with the host binding, for layout checks.
+extern "C" int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task) {
+  if (gs_desc) *gs_desc = sizeof(LcmGsDesc);
+  if (gs_task) *gs_task = sizeof(LcmGsTask);
+  return LCM_OK;
+}
