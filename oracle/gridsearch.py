@@ -429,20 +429,25 @@ def _pointwise_points(reg):
             yield sdi, dai, v
 
 
-def _q_and_f(model, R, reg, t, params, fixed, gdt, Vnext):
+def tabulate(model, R, reg, t, params, fixed, gdt, Vnext):
+    """Evaluate the user's functions of one (regime, period) on the full
+    state x action grid: utility, feasibility, next continuous state, and per
+    target regime its probability and node weights (Q_and_F.py:466-681)."""
     variables, ndim = reg.full_vars()
     shape = reg.full_shape()
     ev = _env(model, reg, t, params, fixed, gdt, variables)
-    U = np.broadcast_to(np.asarray(ev.call("utility"), dtype=np.float64), shape)
+    tab = {"shape": shape, "terminal": reg.terminal, "targets": [], "scalars": []}
+    tab["U"] = np.broadcast_to(np.asarray(ev.call("utility"), dtype=np.float64), shape)
     F = np.ones(shape, dtype=bool)
     for cname, cf in reg.regime.constraints.items():
         F = F & np.broadcast_to(np.asarray(ev.call(cname, cf), dtype=bool), shape)
-    taste = None
+    tab["F"] = F
+    tab["taste"] = None
     if reg.regime.taste_shocks is not None:
-        taste = float(params[reg.name]["taste_shocks"]["scale"])
+        tab["taste"] = float(params[reg.name]["taste_shocks"]["scale"])
     if reg.terminal:
-        return U, F, taste
-    beta = ev.param("koopmans_aggregator", "discount_factor")
+        return tab
+    tab["beta"] = ev.param("koopmans_aggregator", "discount_factor")
     names = list(model.regime_id_class.__categories__)
     age_next = model.ages.period_to_age(t + 1)
     active_next = [n for n in names if _active(R[n].regime, age_next)]
@@ -460,28 +465,16 @@ def _q_and_f(model, R, reg, t, params, fixed, gdt, Vnext):
             P[sdi + dai] = np.asarray(e.call("next_regime", tr), dtype=np.float64)
         else:
             P[sdi + dai + (int(np.asarray(e.call("next_regime", tr))),)] = 1.0
-
-    def on_grid(arr_sd_da):
-        """(sd..., da...) -> broadcast over [sd..., w, da..., c]."""
-        a = np.asarray(arr_sd_da)
-        a = a.reshape(sd_sizes + (1,) + da_sizes + (1,))
-        return np.broadcast_to(a, shape)
-
-    CE = np.zeros(shape)
-    mass = np.zeros(shape)
-    neg = np.zeros(shape, dtype=bool)
     for n in active_next:
         tgt = R[n]
-        p = on_grid(P[..., names.index(n)])
-        mass = mass + p
-        neg = neg | (np.signbit(p) & (p != 0))
+        p = P[..., names.index(n)]
         if tgt.V_shape == ():
-            CE = CE + zero_safe(p, np.float64(Vnext[n]))
+            tab["scalars"].append((n, p, np.float64(Vnext[n])))
             continue
-        # next continuous state on the full grid
         law = reg.regime.state_transitions[tgt.w[0]]
         wn = np.broadcast_to(np.asarray(ev.call(f"next_{tgt.w[0]}", law), dtype=np.float64), shape)
-        coord = coordinate(tgt.w[1], tgt.w_pts, wn)
+        tab["wn"] = wn
+        tab["grid"] = (tgt.w[1], tgt.w_pts)
         # joint weights over the target's discrete/process axes, per (sd, da)
         axes_w = []
         for (s, g) in tgt.sd:
@@ -502,24 +495,55 @@ def _q_and_f(model, R, reg, t, params, fixed, gdt, Vnext):
                     nodes_t, P_t = tgt.proc[s]
                     Wax[sdi + dai] = transition_row(P_t, nodes_t, v[s])
             axes_w.append(Wax)
-        Vn = np.asarray(Vnext[n], dtype=np.float64).reshape(-1, tgt.V_shape[-1])
         node_index = list(np.ndindex(*[a.shape[-1] for a in axes_w])) if axes_w else [()]
+        W = np.ones(sd_sizes + da_sizes + (len(node_index),))
+        for flat, idx in enumerate(node_index):
+            for ax_i, k in enumerate(idx):
+                W[..., flat] = W[..., flat] * axes_w[ax_i][..., k]
+        Vn = np.asarray(Vnext[n], dtype=np.float64).reshape(-1, tgt.V_shape[-1])
+        tab["targets"].append((n, p, W, Vn))
+    return tab
+
+
+def _q_and_f(model, R, reg, t, params, fixed, gdt, Vnext):
+    tab = tabulate(model, R, reg, t, params, fixed, gdt, Vnext)
+    U, F, shape = tab["U"], tab["F"], tab["shape"]
+    if tab["terminal"]:
+        return U, F, tab["taste"]
+    sd_sizes = tuple(len(v) for v in reg.sd_vals)
+    da_sizes = tuple(len(v) for v in reg.da_vals)
+
+    def on_grid(arr_sd_da):
+        """(sd..., da...) -> broadcast over [sd..., w, da..., c]."""
+        a = np.asarray(arr_sd_da).reshape(sd_sizes + (1,) + da_sizes + (1,))
+        return np.broadcast_to(a, shape)
+
+    CE = np.zeros(shape)
+    mass = np.zeros(shape)
+    neg = np.zeros(shape, dtype=bool)
+    for n, p, v in tab["scalars"]:
+        pg = on_grid(p)
+        mass = mass + pg
+        neg = neg | (np.signbit(pg) & (pg != 0))
+        CE = CE + zero_safe(pg, v)
+    for n, p, W, Vn in tab["targets"]:
+        pg = on_grid(p)
+        mass = mass + pg
+        neg = neg | (np.signbit(pg) & (pg != 0))
+        grid, pts = tab["grid"]
+        coord = coordinate(grid, pts, tab["wn"])
         num = np.zeros(shape)
         wsum = np.zeros(shape)
-        for flat, idx in enumerate(node_index):
-            wk = np.ones(sd_sizes + da_sizes)
-            for ax_i, k in enumerate(idx):
-                wk = wk * axes_w[ax_i][..., k]
-            wk_g = on_grid(wk)
-            vk = interp_1d(Vn[flat], coord)
-            num = num + zero_safe(wk_g, vk)
-            wsum = wsum + wk_g
+        for k in range(W.shape[-1]):
+            wk = on_grid(W[..., k])
+            num = num + zero_safe(wk, interp_1d(Vn[k], coord))
+            wsum = wsum + wk
         E = num / np.where(wsum > 0, wsum, 1.0)
-        CE = CE + zero_safe(p, E)
+        CE = CE + zero_safe(pg, E)
     ok = (np.abs(mass - 1.0) <= 1e-3) & ~neg
     CE = CE / np.where(ok, mass, np.nan)
-    Q = U + beta * CE
-    return Q, F, taste
+    Q = U + tab["beta"] * CE
+    return Q, F, tab["taste"]
 
 
 def _max_over_actions(Q, F, reg, taste):

@@ -284,6 +284,25 @@ class PeriodLowering:
         self.params = params
         self.fixed = model.fixed_params
         self.rg = rg
+        self._memo = {}
+        self._age_free_memo = {}
+
+    def _age_free(self, lay, funcs) -> bool:
+        """Whether none of `funcs` reads `age` or `period`, directly or through
+        another regime function (then their tables are the same every period)."""
+        key = (lay.name, tuple(id(f) for f in funcs))
+        if key not in self._age_free_memo:
+            names = set()
+            for f in funcs:
+                names |= _arg_closure(lay, f)
+            self._age_free_memo[key] = not ({"age", "period"} & names)
+        return self._age_free_memo[key]
+
+    def _cached(self, key, lay, funcs, period, compute):
+        k = (key, lay.name, None if self._age_free(lay, funcs) else period)
+        if k not in self._memo:
+            self._memo[k] = compute()
+        return self._memo[k]
 
     def _evaluator(self, lay, variables, period):
         age = self.model.ages.period_to_age(period)
@@ -401,11 +420,37 @@ class PeriodLowering:
             )
         return out
 
+    def _vectorized_ids(self, lay, period, tr):
+        """A deterministic regime transition evaluated on the broadcast
+        (sd, da) grid at once (it is scalar-valued, so broadcasting is exact)."""
+        ax = _Axes(lay, self.rg, full_w=False)
+        variables = {k: v for k, v in ax.vars.items() if k != lay.c_name}
+        if lay.c_name and lay.c_name in _arg_closure(lay, tr):
+            raise UnsupportedModelError(
+                f"regime {lay.name!r}: next_regime depends on the continuous action; not implemented"
+            )
+        out = self._evaluator(lay, variables, period).call("next_regime", tr)
+        if isinstance(out, (Affine, Constraint)):
+            if isinstance(out, Affine) and out.is_const:
+                out = out.b
+            else:
+                raise UnsupportedModelError(
+                    f"regime {lay.name!r}: next_regime depends on the continuous state; not implemented"
+                )
+        ids = ax.to4(np.asarray(out, dtype=np.float64))[:, 0, :, 0]
+        return ids.astype(np.int64)
+
     def regime_probs(self, lay, period):
         """p[sd, d, regime_code] of moving to each regime."""
         n_reg = len(self.model.regime_names)
         p = np.zeros((lay.n_sd, lay.n_d, n_reg))
         tr = lay.regime.transition
+        if not isinstance(tr, MarkovTransition) and not hasattr(tr, "items"):
+            ids = self._vectorized_ids(lay, period, tr)
+            if np.any((ids < 0) | (ids >= n_reg)):
+                raise InvalidParamsError(f"regime {lay.name!r}: next regime id out of range")
+            np.put_along_axis(p, ids[..., None], 1.0, axis=-1)
+            return p
         for i, j, v in self._points(lay):
             if isinstance(tr, dict) or hasattr(tr, "items"):
                 for tname, f in tr.items():
@@ -471,8 +516,11 @@ class PeriodLowering:
     def lower(self, lay: RegimeLayout, period: int) -> PeriodTables:
         model = self.model
         T = model.n_periods
-        U = self.utility(lay, period)
-        F = self.constraints(lay, period)
+        reg = lay.regime
+        U = self._cached("U", lay, [reg.functions["utility"]], period,
+                         lambda: self.utility(lay, period))
+        cons = [f for f in reg.constraints.values() if f is not None]
+        F = self._cached("F", lay, cons, period, lambda: self.constraints(lay, period))
         pt = PeriodTables(regime=lay.name, period=period, terminal=lay.terminal, U=U, F=F)
         if lay.terminal:
             return pt
@@ -491,7 +539,9 @@ class PeriodLowering:
             if not pt.taste_scale > 0:
                 raise InvalidParamsError("taste shock scale must be positive")
         age_next = model.ages.period_to_age(period + 1)
-        p = self.regime_probs(lay, period)
+        tr = reg.transition
+        tr_funcs = list(tr.values()) if hasattr(tr, "items") else [tr]
+        p = self._cached("P", lay, tr_funcs, period, lambda: self.regime_probs(lay, period))
         active_next = [r for r in model.regime_names if model.layouts[r].regime.active(age_next)]
         for r in model.regime_names:
             code = model.regime_code(r)
@@ -520,18 +570,32 @@ class PeriodLowering:
                 if not np.array_equal(self.rg.w_pts[r], self.rg.w_pts[ref.name]) or tl.w_grid.kind != ref.w_grid.kind:
                     raise UnsupportedModelError("target regimes must share one continuous grid")
             pt.next_grid = ref.name
-            pt.A, pt.B = self.next_continuous(lay, period)
+            law = reg.state_transitions.get(lay.w_name)
+            pt.A, pt.B = self._cached("AB", lay, [law] if law is not None else [], period,
+                                      lambda: self.next_continuous(lay, period))
         for r in stateful:
             tl = model.layouts[r]
             pr = p[:, :, model.regime_code(r)]
-            W = self.node_weights(lay, tl, period)
-            wsum = W.sum(axis=-1, keepdims=True)
-            Wn = W / np.where(wsum > 0, wsum, 1.0)
-            M = np.where(pr[..., None] != 0, pr[..., None] * Wn, 0.0) / safe_mass[..., None]
-            pt.targets.append((r, M))
+            laws = []
+            for s in tl.sd_names:
+                f = reg.state_transitions.get(s)
+                if f is not None:
+                    laws += list(f.values()) if hasattr(f, "items") else [f]
+            W = self._cached(("W", r), lay, laws, period,
+                             lambda tl=tl: self.node_weights(lay, tl, period))
+            key = ("M", lay.name, r, id(W), pr.tobytes(), safe_mass.tobytes())
+            if key not in self._memo:
+                wsum = W.sum(axis=-1, keepdims=True)
+                Wn = W / np.where(wsum > 0, wsum, 1.0)
+                self._memo[key] = (np.where(pr[..., None] != 0, pr[..., None] * Wn, 0.0)
+                                   / safe_mass[..., None], W)
+            pt.targets.append((r, self._memo[key][0]))
         for r in scalar:
             pr = p[:, :, model.regime_code(r)]
-            pt.scalars.append((r, np.where(pr != 0, pr, 0.0) / safe_mass))
+            key = ("m", lay.name, r, pr.tobytes(), safe_mass.tobytes())
+            if key not in self._memo:
+                self._memo[key] = np.where(pr != 0, pr, 0.0) / safe_mass
+            pt.scalars.append((r, self._memo[key]))
         if not ok.all():
             # Poison CE where the regime mass is not a distribution: a NaN weight
             # on a zero-valued scalar term turns Q into NaN.

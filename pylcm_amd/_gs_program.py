@@ -1,12 +1,14 @@
 """Device plan of grid-search solves: arena, descriptors, tasks, launches.
 
-A `GsProgram` holds any number of problems (one params set each, possibly of
-different models' grid sizes) for one model structure. It lowers every
-(problem, period, regime) to `PeriodTables`, packs all tables into one device
-arena (identical tables are stored once), writes one `LcmGsDesc` per
-(problem, period, regime) and one task list per period, and then runs backward
-induction as one fused launch per period, optionally captured once as a HIP
-graph over the horizon. This replaces the reference's host loop
+A `GsProgram` holds any number of problems — (model, params) pairs that may
+differ in grid sizes and horizons — lowered to `PeriodTables`. It packs all
+tables into one device arena (identical tables are stored once), writes one
+`LcmGsDesc` per (problem, period, regime) and one task list per backward step,
+and runs backward induction as one fused launch per step, optionally captured
+once as a HIP graph over the horizon. Problems are aligned at their last
+period: step s solves period `T_p - 1 - s` of problem p.
+
+This replaces the reference's host loop
 (`src/_lcm/solution/backward_induction.py:155-275`), which dispatches one
 compiled core per (regime, period) and synchronises after each period.
 """
@@ -20,12 +22,13 @@ from dataclasses import dataclass
 import numpy as np
 
 from pylcm_amd import _native as nat
-from pylcm_amd._build import PeriodLowering, PeriodTables, resolve_grids, table_digest
+from pylcm_amd._build import PeriodLowering, resolve_grids, table_digest
 from pylcm_amd.exceptions import UnsupportedModelError
 
 ALIGN = 256
 DESC_DTYPE = np.dtype(nat.LcmGsDesc)
 TASK_DTYPE = np.dtype(nat.LcmGsTask)
+FLOPS_PER_Q = 10  # budget 2, coordinate 2, interpolation 3, aggregator 2, max 1
 
 
 def _align(x: int) -> int:
@@ -39,17 +42,29 @@ class _Arena:
         self.parts: list[tuple[int, bytes]] = []
         self.size = 0
         self.index: dict[str, int] = {}
+        self.by_id: dict[tuple, int] = {}
+        self._keep: list = []
 
-    def add(self, arr: np.ndarray, dedupe: bool = True) -> int:
-        a = np.ascontiguousarray(arr)
+    def add(self, arr: np.ndarray, dedupe: bool = True, dtype=None) -> int:
+        """Place `arr` (cast to `dtype`) in the arena; the same array object or
+        the same contents are stored once."""
+        if dedupe and dtype is not None:
+            idk = (id(arr), np.dtype(dtype).str)
+            if idk in self.by_id:
+                return self.by_id[idk]
+        a = np.ascontiguousarray(arr if dtype is None else arr.astype(dtype))
         key = table_digest(a) + str(a.dtype) if dedupe else None
         if key is not None and key in self.index:
-            return self.index[key]
-        off = _align(self.size)
-        self.parts.append((off, a.tobytes()))
-        self.size = off + a.nbytes
-        if key is not None:
-            self.index[key] = off
+            off = self.index[key]
+        else:
+            off = _align(self.size)
+            self.parts.append((off, a.tobytes()))
+            self.size = off + a.nbytes
+            if key is not None:
+                self.index[key] = off
+        if dedupe and dtype is not None:
+            self.by_id[(id(arr), np.dtype(dtype).str)] = off
+            self._keep.append(arr)
         return off
 
     def reserve(self, nbytes: int) -> int:
@@ -57,12 +72,10 @@ class _Arena:
         self.size = off + max(int(nbytes), 1)
         return off
 
-    def image(self, upto: int | None = None) -> np.ndarray:
-        n = self.size if upto is None else upto
-        buf = np.zeros(_align(max(n, 1)), dtype=np.uint8)
+    def image(self) -> np.ndarray:
+        buf = np.zeros(_align(max(self.size, 1)), dtype=np.uint8)
         for off, b in self.parts:
-            if off < n:
-                buf[off: off + len(b)] = np.frombuffer(b, dtype=np.uint8)
+            buf[off: off + len(b)] = np.frombuffer(b, dtype=np.uint8)
         return buf
 
 
@@ -90,142 +103,125 @@ def default_tile(n_states_total: int) -> int:
     return 256
 
 
+def lower_problem(model, params, dtype):
+    """(tables by (period, regime), resolved grids) for one params set."""
+    rg = resolve_grids(model.layouts, params, model.fixed_params, dtype)
+    low = PeriodLowering(model, params, rg)
+    tabs = {}
+    for t in range(model.n_periods):
+        age = model.ages.period_to_age(t)
+        for r in model.regime_names:
+            lay = model.layouts[r]
+            if not lay.regime.active(age):
+                continue
+            if lay.is_dcegm:
+                raise UnsupportedModelError("DC-EGM regimes are solved by the DC-EGM program")
+            tabs[(t, r)] = low.lower(lay, t)
+    return tabs, rg
+
+
 class GsProgram:
+    """Packed, device-resident grid-search solves.
+
+    `GsProgram(model, [params, ...], dtype)` solves one model for many params;
+    `GsProgram.of([(model, params), ...], dtype)` mixes models.
+    """
+
     def __init__(self, model, params_list, dtype, *, tile: int | None = None, ctx=None,
-                 tables=None):
-        self.model = model
+                 lowered=None):
+        self._init([(model, p) for p in params_list], dtype, tile=tile, ctx=ctx, lowered=lowered)
+
+    @classmethod
+    def of(cls, problems, dtype, *, tile=None, ctx=None, lowered=None):
+        self = cls.__new__(cls)
+        self._init(list(problems), dtype, tile=tile, ctx=ctx, lowered=lowered)
+        return self
+
+    def _init(self, problems, dtype, *, tile, ctx, lowered):
         self.dt = np.dtype(dtype)
         if self.dt not in (np.dtype(np.float32), np.dtype(np.float64)):
             raise UnsupportedModelError(f"dtype {self.dt} is not supported")
-        self.T = model.n_periods
+        self.models = [m for m, _ in problems]
         self.ctx = ctx if ctx is not None else nat.Context()
         t0 = time.perf_counter()
-        lowered = tables if tables is not None else [
-            lower_problem(model, p, self.dt) for p in params_list
-        ]
-        self.problems = [tabs for tabs, _ in lowered]
-        self._rg = [rg for _, rg in lowered]
+        low = lowered if lowered is not None else [lower_problem(m, p, self.dt) for m, p in problems]
+        self.problems = [tabs for tabs, _ in low]
+        self._rg = [rg for _, rg in low]
         self.lower_seconds = time.perf_counter() - t0
+        self.n_steps = max(m.n_periods for m in self.models)
         n_states = sum(
-            model.layouts[r].n_sd * model.layouts[r].n_w
-            for tabs in self.problems for (t, r) in tabs if t == 0
+            m.layouts[r].n_sd * m.layouts[r].n_w
+            for m, tabs in zip(self.models, self.problems) for (t, r) in tabs
+            if t == m.n_periods - 2
         ) or 1
         self.tile = tile or default_tile(n_states)
         self._graph = None
+        t0 = time.perf_counter()
         self._pack()
+        self.pack_seconds = time.perf_counter() - t0
+
+    @property
+    def model(self):
+        return self.models[0]
+
+    def step_of(self, p: int, t: int) -> int:
+        return self.models[p].n_periods - 1 - t
 
     # -- packing ---------------------------------------------------------------
     def _pack(self):
-        model, dt, T = self.model, self.dt, self.T
+        dt = self.dt
         isz = dt.itemsize
+        S = self.n_steps
         ar = _Arena()
         zero_off = ar.add(np.zeros(1, dtype=dt))
         self.outs: list[dict] = []
         # outputs are reserved first so that descriptors can point at them
-        for tabs in self.problems:
+        for model, tabs in zip(self.models, self.problems):
             o = {}
             for (t, r), pt in tabs.items():
                 lay = model.layouts[r]
                 n = lay.n_sd * lay.n_w
-                pol_n = n * (lay.n_d if pt.taste_scale is not None else 1)
+                taste = pt.taste_scale is not None
+                pol_n = n * (lay.n_d if taste else 1)
                 o[(t, r)] = _Out(ar.reserve(n * isz), ar.reserve(pol_n * 4), lay.V_shape,
-                                 lay.V_shape + ((lay.n_d,) if pt.taste_scale is not None else ()))
+                                 lay.V_shape + ((lay.n_d,) if taste else ()))
             self.outs.append(o)
         self.out_end = ar.size
         descs = []
-        desc_period = []
-        tasks_by_t = [[] for _ in range(T)]
-        wdep_by_t = [False] * T
-        gmax_by_t = [1] * T
-        for p, (tabs, rg) in enumerate(zip(self.problems, self._rg)):
-            for (t, r), pt in sorted(tabs.items(), key=lambda kv: (kv[0][0], model.regime_code(kv[0][1]))):
+        tasks = [[] for _ in range(S)]
+        wdep = [False] * S
+        gmax = [1] * S
+        self.flops_by_step = [0.0] * S
+        self.bytes_by_step = [0.0] * S
+        for p, (model, tabs, rg) in enumerate(zip(self.models, self.problems, self._rg)):
+            order = sorted(tabs.items(), key=lambda kv: (-kv[0][0], model.regime_code(kv[0][1])))
+            for (t, r), pt in order:
+                s = self.step_of(p, t)
                 lay = model.layouts[r]
-                d = {}
-                out = self.outs[p][(t, r)]
-                d["V_out"] = out.v_off
-                d["pol_out"] = out.p_off
-                d["w_pts"] = ar.add(rg.w_pts[r].astype(dt)) if lay.w_grid is not None else zero_off
-                d["U"] = ar.add(pt.U.astype(dt))
-                d["U_ssd"], d["U_sw"] = _strides(pt.U)
-                wdep = d["U_sw"] != 0
-                if len(pt.F) > nat.MAX_CONSTR:
-                    raise UnsupportedModelError(f"at most {nat.MAX_CONSTR} affine constraints per regime")
-                fa, fb, fa_s, fb_s, strict = [], [], [], [], 0
-                for k, (Fa, Fb, st) in enumerate(pt.F):
-                    fa.append(ar.add(Fa.astype(dt)))
-                    fb.append(ar.add(Fb.astype(dt)))
-                    fa_s.append(_strides(Fa))
-                    fb_s.append(_strides(Fb))
-                    wdep |= fa_s[-1][1] != 0 or fb_s[-1][1] != 0
-                    strict |= int(st) << k
-                d["Fa"], d["Fb"], d["Fa_s"], d["Fb_s"] = fa, fb, fa_s, fb_s
-                d["n_constr"], d["strict_mask"] = len(pt.F), strict
-                d["terminal"] = int(pt.terminal)
-                d["n_sd"], d["n_w"], d["n_d"], d["n_c"] = lay.n_sd, lay.n_w, lay.n_d, lay.n_c
-                d["has_taste"] = int(pt.taste_scale is not None)
-                d["taste_scale"] = pt.taste_scale or 0.0
-                if pt.taste_scale is not None and lay.n_d > nat.MAX_TASTE_D:
-                    raise UnsupportedModelError(f"taste shocks over more than {nat.MAX_TASTE_D} discrete actions")
-                d["beta"] = pt.beta
-                d["Vn"], d["M"], d["nj"], d["Vs"], d["ms"] = [], [], [], [], []
-                d["g_kind"], d["g_n"], d["g_start"], d["g_stop"], d["g_pts"] = 0, 0, 0.0, 0.0, zero_off
-                d["A"] = d["B"] = zero_off
-                d["A_ssd"] = d["A_sw"] = d["B_ssd"] = d["B_sw"] = 0
-                # shape checks of everything the kernel indexes (see LcmGsDesc)
-                for tab in [pt.U] + [x for Fa, Fb, _ in pt.F for x in (Fa, Fb)]:
-                    assert tab.ndim == 4 and tab.shape[2:] == (lay.n_d, lay.n_c), tab.shape
-                    assert tab.shape[0] in (1, lay.n_sd) and tab.shape[1] in (1, lay.n_w), tab.shape
-                if not pt.terminal:
-                    if len(pt.targets) > nat.MAX_TARGETS or len(pt.scalars) > nat.MAX_SCALAR:
-                        raise UnsupportedModelError("too many target regimes for the kernel descriptor")
-                    for name, M in pt.targets:
-                        tl = model.layouts[name]
-                        assert M.shape == (lay.n_sd, lay.n_d, tl.n_sd), (M.shape, tl.n_sd)
-                        assert tl.n_w == len(rg.w_pts[pt.next_grid])
-                    for name, m in pt.scalars:
-                        assert m.shape == (lay.n_sd, lay.n_d)
-                        assert name is None or model.layouts[name].V_shape == ()
-                    if pt.targets:
-                        for tab in (pt.A, pt.B):
-                            assert tab.ndim == 4 and tab.shape[2:] == (lay.n_d, lay.n_c), tab.shape
-                            assert tab.shape[0] in (1, lay.n_sd) and tab.shape[1] in (1, lay.n_w)
-                    for name, M in pt.targets:
-                        d["Vn"].append(self.outs[p][(t + 1, name)].v_off)
-                        d["M"].append(ar.add(M.astype(dt)))
-                        d["nj"].append(M.shape[-1])
-                    for name, m in pt.scalars:
-                        d["Vs"].append(self.outs[p][(t + 1, name)].v_off if name is not None else zero_off)
-                        d["ms"].append(ar.add(m.astype(dt)))
-                    if pt.targets:
-                        glay = model.layouts[pt.next_grid]
-                        gpts = rg.w_pts[pt.next_grid]
-                        d["g_kind"] = int(glay.w_grid.kind)
-                        d["g_n"] = len(gpts)
-                        d["g_start"], d["g_stop"] = float(glay.w_grid.start), float(glay.w_grid.stop)
-                        d["g_pts"] = ar.add(gpts.astype(dt))
-                        d["A"] = ar.add(pt.A.astype(dt))
-                        d["B"] = ar.add(pt.B.astype(dt))
-                        d["A_ssd"], d["A_sw"] = _strides(pt.A)
-                        d["B_ssd"], d["B_sw"] = _strides(pt.B)
-                        wdep |= d["A_sw"] != 0 or d["B_sw"] != 0
-                        gmax_by_t[t] = max(gmax_by_t[t], d["g_n"])
+                d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
                 di = len(descs)
                 descs.append(d)
-                desc_period.append(t)
-                wdep_by_t[t] |= wdep
+                wdep[s] |= d["wdep"]
+                if d["g_n"] > 0:
+                    gmax[s] = max(gmax[s], d["g_n"])
+                n_states = lay.n_sd * lay.n_w
+                n_q = n_states * lay.n_d * lay.n_c
+                ev_flops = 2.0 * lay.n_sd * lay.n_d * sum(d["nj"]) * max(d["g_n"], 0)
+                self.flops_by_step[s] += FLOPS_PER_Q * n_q + ev_flops
+                # algorithmic HBM bytes: next V rows read, V and policy written
+                n_next = sum(nj * d["g_n"] for nj in d["nj"])
+                self.bytes_by_step[s] += isz * (n_next + n_states) + 4 * n_states
                 for sd in range(lay.n_sd):
                     for w0 in range(0, lay.n_w, self.tile):
-                        tasks_by_t[t].append((di, sd, w0))
+                        tasks[s].append((di, sd, w0))
         self.n_desc = len(descs)
         self.desc_off = ar.reserve(self.n_desc * DESC_DTYPE.itemsize)
         self.task_off, self.n_tasks = [], []
-        task_arrays = []
-        for t in range(T):
-            tl = tasks_by_t[t]
+        for s in range(S):
+            tl = tasks[s]
             if not tl:
                 self.task_off.append(0)
                 self.n_tasks.append(0)
-                task_arrays.append(None)
                 continue
             n = len(tl)
             npad = (n + 7) // 8 * 8
@@ -237,66 +233,139 @@ class GsProgram:
             arr["desc"][n:] = -1
             self.task_off.append(ar.add(arr, dedupe=False))
             self.n_tasks.append(npad)
-            task_arrays.append(arr)
-        self.wdep_by_t = wdep_by_t
-        self.lds_by_t = [int(2 * g * self.dt.itemsize + 256 * (self.dt.itemsize + 8) + 64) for g in gmax_by_t]
+        self.wdep_by_step = wdep
+        self.lds_by_step = [int(2 * g * isz + 256 * (isz + 8) + 64) for g in gmax]
         self.total_bytes = _align(ar.size)
         self.buf = nat.DeviceBuffer(self.total_bytes)
-        base = self.buf.ptr
-        self.base = base
-        D = np.zeros(self.n_desc, dtype=DESC_DTYPE)
-
-        def ptr(off):
-            return base + off
-
-        for i, d in enumerate(descs):
-            rec = D[i]
-            rec["V_out"] = ptr(d["V_out"])
-            rec["pol_out"] = ptr(d["pol_out"])
-            rec["w_pts"] = ptr(d["w_pts"])
-            rec["U"] = ptr(d["U"])
-            rec["A"] = ptr(d["A"])
-            rec["B"] = ptr(d["B"])
-            for k in range(nat.MAX_CONSTR):
-                if k < d["n_constr"]:
-                    rec["Fa"][k] = ptr(d["Fa"][k])
-                    rec["Fb"][k] = ptr(d["Fb"][k])
-                    rec["Fa_ssd"][k], rec["Fa_sw"][k] = d["Fa_s"][k]
-                    rec["Fb_ssd"][k], rec["Fb_sw"][k] = d["Fb_s"][k]
-            for k, (v, m, nj) in enumerate(zip(d["Vn"], d["M"], d["nj"])):
-                rec["Vn"][k] = ptr(v)
-                rec["M"][k] = ptr(m)
-                rec["n_sd_next"][k] = nj
-            for k, (v, m) in enumerate(zip(d["Vs"], d["ms"])):
-                rec["Vs"][k] = ptr(v)
-                rec["ms"][k] = ptr(m)
-            rec["g_pts"] = ptr(d["g_pts"])
-            for f in ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
-                      "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
-                      "beta", "taste_scale", "g_start", "g_stop"):
-                rec[f] = d[f]
-            rec["n_targets"] = len(d["Vn"])
-            rec["n_scalar"] = len(d["Vs"])
+        self.base = self.buf.ptr
+        D = self._desc_records(descs)
         ar.parts.append((self.desc_off, D.tobytes()))
         image = ar.image()
         self.host_image_bytes = image.nbytes
-        # upload everything (outputs are overwritten by the kernels)
-        nat.h2d(self.ctx, base, image)
+        nat.h2d(self.ctx, self.base, image)
         self.out_host = np.zeros(self.out_end, dtype=np.uint8)
 
+    def _describe(self, model, p, t, r, pt, rg, ar, zero_off):
+        dt = self.dt
+        lay = model.layouts[r]
+        d = {}
+        out = self.outs[p][(t, r)]
+        d["V_out"] = out.v_off
+        d["pol_out"] = out.p_off
+        d["w_pts"] = ar.add(rg.w_pts[r], dtype=dt) if lay.w_grid is not None else zero_off
+        # shape checks of everything the kernel indexes (see LcmGsDesc)
+        for tab in [pt.U] + [x for Fa, Fb, _ in pt.F for x in (Fa, Fb)]:
+            assert tab.ndim == 4 and tab.shape[2:] == (lay.n_d, lay.n_c), tab.shape
+            assert tab.shape[0] in (1, lay.n_sd) and tab.shape[1] in (1, lay.n_w), tab.shape
+        d["U"] = ar.add(pt.U, dtype=dt)
+        d["U_ssd"], d["U_sw"] = _strides(pt.U)
+        wdep = d["U_sw"] != 0
+        if len(pt.F) > nat.MAX_CONSTR:
+            raise UnsupportedModelError(f"at most {nat.MAX_CONSTR} affine constraints per regime")
+        fa, fb, fa_s, fb_s, strict = [], [], [], [], 0
+        for k, (Fa, Fb, st) in enumerate(pt.F):
+            fa.append(ar.add(Fa, dtype=dt))
+            fb.append(ar.add(Fb, dtype=dt))
+            fa_s.append(_strides(Fa))
+            fb_s.append(_strides(Fb))
+            wdep |= fa_s[-1][1] != 0 or fb_s[-1][1] != 0
+            strict |= int(st) << k
+        d["Fa"], d["Fb"], d["Fa_s"], d["Fb_s"] = fa, fb, fa_s, fb_s
+        d["n_constr"], d["strict_mask"] = len(pt.F), strict
+        d["terminal"] = int(pt.terminal)
+        d["n_sd"], d["n_w"], d["n_d"], d["n_c"] = lay.n_sd, lay.n_w, lay.n_d, lay.n_c
+        d["has_taste"] = int(pt.taste_scale is not None)
+        d["taste_scale"] = pt.taste_scale or 0.0
+        if pt.taste_scale is not None and lay.n_d > nat.MAX_TASTE_D:
+            raise UnsupportedModelError(f"taste shocks over more than {nat.MAX_TASTE_D} discrete actions")
+        d["beta"] = pt.beta
+        d["Vn"], d["M"], d["nj"], d["Vs"], d["ms"] = [], [], [], [], []
+        d["g_kind"], d["g_n"], d["g_start"], d["g_stop"], d["g_pts"] = 0, 0, 0.0, 0.0, zero_off
+        d["A"] = d["B"] = zero_off
+        d["A_ssd"] = d["A_sw"] = d["B_ssd"] = d["B_sw"] = 0
+        if not pt.terminal:
+            if len(pt.targets) > nat.MAX_TARGETS or len(pt.scalars) > nat.MAX_SCALAR:
+                raise UnsupportedModelError("too many target regimes for the kernel descriptor")
+            for name, M in pt.targets:
+                tl = model.layouts[name]
+                assert M.shape == (lay.n_sd, lay.n_d, tl.n_sd), (M.shape, tl.n_sd)
+                assert tl.n_w == len(rg.w_pts[pt.next_grid])
+                d["Vn"].append(self.outs[p][(t + 1, name)].v_off)
+                d["M"].append(ar.add(M, dtype=dt))
+                d["nj"].append(M.shape[-1])
+            for name, m in pt.scalars:
+                assert m.shape == (lay.n_sd, lay.n_d)
+                assert name is None or model.layouts[name].V_shape == ()
+                d["Vs"].append(self.outs[p][(t + 1, name)].v_off if name is not None else zero_off)
+                d["ms"].append(ar.add(m, dtype=dt))
+            if pt.targets:
+                for tab in (pt.A, pt.B):
+                    assert tab.ndim == 4 and tab.shape[2:] == (lay.n_d, lay.n_c), tab.shape
+                    assert tab.shape[0] in (1, lay.n_sd) and tab.shape[1] in (1, lay.n_w)
+                glay = model.layouts[pt.next_grid]
+                gpts = rg.w_pts[pt.next_grid]
+                d["g_kind"] = int(glay.w_grid.kind)
+                d["g_n"] = len(gpts)
+                d["g_start"], d["g_stop"] = float(glay.w_grid.start), float(glay.w_grid.stop)
+                d["g_pts"] = ar.add(gpts, dtype=dt)
+                d["A"] = ar.add(pt.A, dtype=dt)
+                d["B"] = ar.add(pt.B, dtype=dt)
+                d["A_ssd"], d["A_sw"] = _strides(pt.A)
+                d["B_ssd"], d["B_sw"] = _strides(pt.B)
+                wdep |= d["A_sw"] != 0 or d["B_sw"] != 0
+        d["wdep"] = bool(wdep)
+        return d
+
+    def _desc_records(self, descs):
+        """Descriptor records, filled column by column."""
+        base = self.base
+        n = len(descs)
+        D = np.zeros(n, dtype=DESC_DTYPE)
+        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts"):
+            D[f] = base + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
+        for f in ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
+                  "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
+                  "beta", "taste_scale", "g_start", "g_stop"):
+            D[f] = np.array([d[f] for d in descs], dtype=D.dtype[f])
+        D["n_targets"] = np.array([len(d["Vn"]) for d in descs], dtype=np.int32)
+        D["n_scalar"] = np.array([len(d["Vs"]) for d in descs], dtype=np.int32)
+
+        def fill(field, key, k, transform):
+            rows = [i for i, d in enumerate(descs) if len(d[key]) > k]
+            if rows:
+                D[field][rows, k] = [transform(descs[i][key][k]) for i in rows]
+
+        for k in range(nat.MAX_CONSTR):
+            fill("Fa", "Fa", k, lambda v: base + v)
+            fill("Fb", "Fb", k, lambda v: base + v)
+            fill("Fa_ssd", "Fa_s", k, lambda v: v[0])
+            fill("Fa_sw", "Fa_s", k, lambda v: v[1])
+            fill("Fb_ssd", "Fb_s", k, lambda v: v[0])
+            fill("Fb_sw", "Fb_s", k, lambda v: v[1])
+        for k in range(nat.MAX_TARGETS):
+            fill("Vn", "Vn", k, lambda v: base + v)
+            fill("M", "M", k, lambda v: base + v)
+            fill("n_sd_next", "nj", k, lambda v: v)
+        for k in range(nat.MAX_SCALAR):
+            fill("Vs", "Vs", k, lambda v: base + v)
+            fill("ms", "ms", k, lambda v: base + v)
+        return D
+
     # -- execution -------------------------------------------------------------
-    def _record(self):
+    def _launch_step(self, s: int):
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
-        for t in reversed(range(self.T)):
-            if not self.n_tasks[t]:
-                continue
-            st = L.lcm_gs_launch_ex(
-                self.ctx.handle, code, C.c_void_p(self.base + self.desc_off),
-                C.c_void_p(self.base + self.task_off[t]), self.n_tasks[t], self.tile,
-                C.c_size_t(self.lds_by_t[t]), int(self.wdep_by_t[t]),
-            )
-            nat._check(st, f"lcm_gs_launch (period {t})")
+        st = L.lcm_gs_launch_ex(
+            self.ctx.handle, code, C.c_void_p(self.base + self.desc_off),
+            C.c_void_p(self.base + self.task_off[s]), self.n_tasks[s], self.tile,
+            C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]),
+        )
+        nat._check(st, f"lcm_gs_launch (step {s})")
+
+    def _record(self):
+        for s in range(self.n_steps):
+            if self.n_tasks[s]:
+                self._launch_step(s)
 
     def capture_graph(self):
         if self._graph is None:
@@ -310,6 +379,20 @@ class GsProgram:
             self._record()
         if sync:
             self.ctx.sync()
+
+    def time_launches(self):
+        """Per-step kernel durations in ms, from HIP events on the program's stream."""
+        evs = [nat.Event() for _ in range(self.n_steps + 1)]
+        ms = []
+        steps = [s for s in range(self.n_steps) if self.n_tasks[s]]
+        evs[0].record(self.ctx)
+        for i, s in enumerate(steps):
+            self._launch_step(s)
+            evs[i + 1].record(self.ctx)
+        self.ctx.sync()
+        for i, s in enumerate(steps):
+            ms.append((s, evs[i].elapsed_ms(evs[i + 1])))
+        return ms
 
     def fetch(self):
         """Copy all V and policy outputs to the host."""
@@ -327,20 +410,3 @@ class GsProgram:
             P = np.frombuffer(h, dtype=np.int32, count=pn, offset=o.p_off).reshape(o.pol_shape).copy()
             out.setdefault(t, {})[r] = (V, P)
         return out
-
-
-def lower_problem(model, params, dtype):
-    """(tables by (period, regime), resolved grids) for one params set."""
-    rg = resolve_grids(model.layouts, params, model.fixed_params, dtype)
-    low = PeriodLowering(model, params, rg)
-    tabs = {}
-    for t in range(model.n_periods):
-        age = model.ages.period_to_age(t)
-        for r in model.regime_names:
-            lay = model.layouts[r]
-            if not lay.regime.active(age):
-                continue
-            if lay.is_dcegm:
-                raise UnsupportedModelError("DC-EGM regimes are solved by the DC-EGM program")
-            tabs[(t, r)] = low.lower(lay, t)
-    return tabs, rg
