@@ -120,8 +120,8 @@ int lcm_memset_async(lcm_ctx* ctx, void* dst, int value, size_t bytes);
  * Fused grid-search period: one launch evaluates every task (any mix of
  * problems, regimes and grid sizes). Replaces `_GridSearchPeriodKernel.__call__`
  * (src/_lcm/solution/grid_search.py:192-212) for all regimes of a period.
- * tile = states per workgroup (64, 128 or 256); lds_bytes = dynamic LDS, at
- * least sizeof(T) * 2 * max(g_n) + 2048.
+ * tile = states per workgroup (64, 128 or 256); lds_bytes = dynamic LDS as
+ * returned by lcm_gs_lds_bytes for the launch's descriptors.
  */
 int lcm_gs_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                   const LcmGsTask* d_tasks, int n_tasks, int tile,
@@ -139,6 +139,10 @@ int lcm_event_create(void** ev);
 int lcm_event_destroy(void* ev);
 int lcm_event_record(lcm_ctx* ctx, void* ev);
 int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms);
+
+/* Dynamic LDS bytes lcm_gs_launch needs when the launch's largest next-state
+ * grid has max_g points and its descriptors have at most max_constr rows. */
+size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr);
 
 /* sizeof the structs above, so a binding can check its mirror of the layout. */
 int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task);

@@ -191,6 +191,7 @@ class GsProgram:
         tasks = [[] for _ in range(S)]
         wdep = [False] * S
         gmax = [1] * S
+        kmax = [0] * S
         self.flops_by_step = [0.0] * S
         self.bytes_by_step = [0.0] * S
         for p, (model, tabs, rg) in enumerate(zip(self.models, self.problems, self._rg)):
@@ -202,6 +203,7 @@ class GsProgram:
                 di = len(descs)
                 descs.append(d)
                 wdep[s] |= d["wdep"]
+                kmax[s] = max(kmax[s], d["n_constr"])
                 if d["g_n"] > 0:
                     gmax[s] = max(gmax[s], d["g_n"])
                 n_states = lay.n_sd * lay.n_w
@@ -234,7 +236,9 @@ class GsProgram:
             self.task_off.append(ar.add(arr, dedupe=False))
             self.n_tasks.append(npad)
         self.wdep_by_step = wdep
-        self.lds_by_step = [int(2 * g * isz + 256 * (isz + 8) + 64) for g in gmax]
+        code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
+        L = nat.load_library()
+        self.lds_by_step = [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
         self.total_bytes = _align(ar.size)
         self.buf = nat.DeviceBuffer(self.total_bytes)
         self.base = self.buf.ptr

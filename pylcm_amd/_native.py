@@ -89,6 +89,7 @@ _SIGS = {
     "lcm_event_record": ([vp, vp], C.c_int),
     "lcm_event_elapsed": ([vp, vp, C.POINTER(C.c_float)], C.c_int),
     "lcm_abi_sizes": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+    "lcm_gs_lds_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),
 }
 
 _lock = threading.Lock()

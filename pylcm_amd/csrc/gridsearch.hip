@@ -1,11 +1,11 @@
 // Fused grid-search period kernel for gfx950.
 //
-// One launch evaluates a whole period of backward induction for any number of
-// (problem, regime) descriptors: for every state it contracts the next-period
-// value rows into EV, interpolates EV at the next continuous state of every
-// action, adds utility, masks infeasible actions and reduces to the max and the
-// first argmax. It stands in for the compiled core
-// `max_Q_over_a(Q_and_F)` (src/_lcm/regime_building/max_Q_over_a.py:30-191,
+// One launch evaluates a whole backward step for any number of (problem,
+// regime) descriptors: for every state it contracts the next-period value rows
+// into EV, interpolates EV at the next continuous state of every action, adds
+// utility, masks infeasible actions and reduces to the max and the first
+// argmax. It stands in for the compiled core `max_Q_over_a(Q_and_F)`
+// (src/_lcm/regime_building/max_Q_over_a.py:30-191,
 // src/_lcm/regime_building/Q_and_F.py:58-170), the interpolation
 // `map_coordinates` (src/_lcm/regime_building/ndimage.py:28-109) on the grid
 // coordinates of src/_lcm/grids/coordinates.py:30-167, the expectation
@@ -13,13 +13,16 @@
 // and, with taste shocks, `logsum_and_softmax` (src/_lcm/logsum.py:32-64).
 //
 // Work decomposition: a workgroup of 256 threads owns TILE consecutive states
-// of one (descriptor, discrete state); 256/TILE waves split the continuous
-// action range and are merged through LDS at the end. Lanes map to states, so
-// every per-action operand (utility, budget coefficients, constraint rows) is
-// wave-uniform and is read through the scalar cache, and the only per-lane
-// memory traffic is the two-corner gather from the EV row staged in LDS.
-// Blocks are remapped so that consecutive tasks (same problem, same V rows)
-// land on the same XCD and share its L2.
+// of one (descriptor, discrete state). Lanes map to states; the 256/TILE waves
+// that share a state interleave over the actions and are merged through LDS
+// at the end (ties keep the smaller flat index, so the result is the first
+// argmax whatever the interleave). Per discrete action the block stages
+//   * the contracted EV row (next-period V rows weighted by M), and
+//   * chunks of CH actions' operands {A, B, U} and constraint rows {Fa, Fb}
+// in LDS. The action loop index is wave-uniform, so every per-action operand is
+// an LDS broadcast read; the only per-lane LDS traffic is the two-corner gather
+// from the EV row (one ds_read2). Blocks are remapped so that consecutive tasks
+// (same problem, same V rows) land on the same XCD and share its L2.
 
 #include <type_traits>
 
@@ -29,6 +32,7 @@ namespace lcm {
 
 constexpr int kBlock = 256;
 constexpr int kXcds = 8;
+constexpr int kActChunk = 256;  // actions staged in LDS at a time
 
 template <typename T>
 struct Grid1D {
@@ -102,11 +106,45 @@ __device__ __forceinline__ void merge(T& vb, Best& b, T vk, const Best& k) {
   }
 }
 
+template <typename T>
+struct alignas(sizeof(T) * 4) Act {
+  T A, B, U, pad;
+};
+
+template <typename T>
+struct alignas(sizeof(T) * 2) Con {
+  T a, b;
+};
+
+__device__ __forceinline__ size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// Merge the SPLIT partial results of each state into the chunk-0 thread.
+template <typename T, int TILE>
+__device__ __forceinline__ void block_merge(T& best, Best& bb, T* red_v, int* red_i, int* red_f,
+                                            int tid, int ls, int grp) {
+  constexpr int SPLIT = kBlock / TILE;
+  if constexpr (SPLIT > 1) {
+    __syncthreads();
+    red_v[tid] = best;
+    red_i[tid] = bb.idx;
+    red_f[tid] = int(bb.seen) | (int(bb.nan) << 1);
+    __syncthreads();
+    if (grp == 0) {
+      for (int k = 1; k < SPLIT; ++k) {
+        const int j = ls + k * TILE;
+        Best o{red_i[j], (red_f[j] & 1) != 0, (red_f[j] & 2) != 0};
+        merge(best, bb, red_v[j], o);
+      }
+    }
+  }
+}
+
 template <typename T, int TILE, bool WDEP>
 __global__ void __launch_bounds__(kBlock)
 gs_period_kernel(const LcmGsDesc* __restrict__ descs,
                  const LcmGsTask* __restrict__ tasks, int n_tasks) {
   constexpr int SPLIT = kBlock / TILE;
+  constexpr int CH = kActChunk;
   extern __shared__ __align__(16) unsigned char smem[];
 
   // XCD-aware remap: hardware dispatches block b to XCD b % 8; give each XCD a
@@ -120,21 +158,29 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const LcmGsDesc* __restrict__ D = descs + tk.desc;
   const int tid = threadIdx.x;
   const int ls = tid % TILE;
-  const int chunk = tid / TILE;
+  // the waves sharing a state interleave over actions; grp is wave-uniform
+  const int grp = __builtin_amdgcn_readfirstlane(tid / TILE);
   const int n_w = D->n_w, n_d = D->n_d, n_c = D->n_c, sd = tk.sd;
   const int wi = tk.w0 + ls;
   const bool valid = wi < n_w;
   const int wr = valid ? wi : 0;
-  const T w = (D->w_pts != nullptr) ? static_cast<const T*>(D->w_pts)[wr] : T(0);
+  const T w = static_cast<const T*>(D->w_pts)[wr];
   const bool terminal = D->terminal != 0;
   const bool taste = D->has_taste != 0;
   // a stateful continuation needs a next-state grid with at least one point
   const bool interp = !terminal && D->n_targets > 0 && D->g_n > 0;
   const int g_n = interp ? D->g_n : 0;
+  const int n_constr = D->n_constr;
+  const int strict_mask = D->strict_mask;
 
+  // LDS carve (the host sizes the launch for the largest g_n and n_constr)
+  const size_t g_bytes = align16(size_t(g_n > 0 ? g_n : 1) * sizeof(T));
   T* ev = reinterpret_cast<T*>(smem);
-  T* gp = ev + (g_n > 0 ? g_n : 1);
-  T* red_v = gp + (g_n > 0 ? g_n : 1);
+  T* gp = reinterpret_cast<T*>(smem + g_bytes);
+  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
+  Con<T>* con = reinterpret_cast<Con<T>*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
+  T* red_v = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>) +
+                                  size_t(n_constr) * CH * sizeof(Con<T>));
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
   int* red_f = red_i + kBlock;
 
@@ -156,15 +202,9 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   }
 
   const T beta = T(D->beta);
-  const int per_chunk = (n_c + SPLIT - 1) / SPLIT;
-  const int cb = min(chunk * per_chunk, n_c);
-  const int ce = min(cb + per_chunk, n_c);
-
   const int64_t wofs_u = WDEP ? int64_t(wr) * D->U_sw : 0;
   const int64_t wofs_a = WDEP ? int64_t(wr) * D->A_sw : 0;
   const int64_t wofs_b = WDEP ? int64_t(wr) * D->B_sw : 0;
-  const int n_constr = D->n_constr;
-  const int strict_mask = D->strict_mask;
 
   T best = -INFINITY;
   Best bb{0, false, false};
@@ -178,7 +218,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     bool finite_row = true;
     if (!terminal) {
       __syncthreads();  // previous readers of ev are done
-      // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k]   (FP32/FP64 accumulate)
+      // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k]   (accumulated in T)
       int local_nonfinite = 0;
       for (int k = tid; k < g_n; k += kBlock) {
         T acc = T(0);
@@ -202,58 +242,85 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
       }
     }
 
-    const T* __restrict__ Ur = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
-    const T* __restrict__ Ar = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
-    const T* __restrict__ Br = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
+    const T* __restrict__ Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
+    const T* __restrict__ Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
+    const T* __restrict__ Bg = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
 
-    for (int c = cb; c < ce; ++c) {
-      bool feas = valid;
-      for (int k = 0; k < n_constr; ++k) {
-        const T* Fa = static_cast<const T*>(D->Fa[k]);
-        const T* Fb = static_cast<const T*>(D->Fb[k]);
-        const int64_t ia = int64_t(sd) * D->Fa_ssd[k] + (WDEP ? int64_t(wr) * D->Fa_sw[k] : 0) + int64_t(d) * n_c + c;
-        const int64_t ib = int64_t(sd) * D->Fb_ssd[k] + (WDEP ? int64_t(wr) * D->Fb_sw[k] : 0) + int64_t(d) * n_c + c;
-        const T f = Fa[ia] * w + Fb[ib];
-        feas = feas && (((strict_mask >> k) & 1) ? (f > T(0)) : (f >= T(0)));
+    for (int c0 = 0; c0 < n_c; c0 += CH) {
+      const int cn = min(CH, n_c - c0);
+      if constexpr (!WDEP) {
+        __syncthreads();  // previous chunk consumed
+        for (int i = tid; i < cn; i += kBlock) {
+          Act<T> a;
+          a.U = Ug[c0 + i];
+          a.A = interp ? Ag[c0 + i] : T(0);
+          a.B = interp ? Bg[c0 + i] : T(0);
+          a.pad = T(0);
+          act[i] = a;
+          for (int k = 0; k < n_constr; ++k) {
+            const T* Fa = static_cast<const T*>(D->Fa[k]) + int64_t(sd) * D->Fa_ssd[k] + int64_t(d) * n_c;
+            const T* Fb = static_cast<const T*>(D->Fb[k]) + int64_t(sd) * D->Fb_ssd[k] + int64_t(d) * n_c;
+            Con<T> f;
+            f.a = Fa[c0 + i];
+            f.b = Fb[c0 + i];
+            con[k * CH + i] = f;
+          }
+        }
+        __syncthreads();
       }
-      if (__ballot(feas) == 0) continue;  // no lane of this wave can take action c
-      T q;
-      if (terminal) {
-        q = Ur[c];
-      } else if (interp) {
-        const T x = Ar[c] * w + Br[c];
-        const T v = interp_row(ev, g_n, coordinate(g, x), finite_row);
-        q = Ur[c] + beta * (cs + v);
-      } else {
-        q = Ur[c] + beta * cs;
-      }
-      if (feas) {
-        if (q != q) {
-          bb.nan = true;
-        } else if (!bb.seen || q > best) {
-          best = q;
-          bb.idx = d * n_c + c;
-          bb.seen = true;
+      for (int i = grp; i < cn; i += SPLIT) {
+        const int c = c0 + i;
+        T Uc, Ac, Bc;
+        bool feas = valid;
+        if constexpr (!WDEP) {
+          const Act<T> a = act[i];
+          Uc = a.U;
+          Ac = a.A;
+          Bc = a.B;
+          for (int k = 0; k < n_constr; ++k) {
+            const Con<T> f = con[k * CH + i];
+            const T v = f.a * w + f.b;
+            feas = feas && (((strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
+          }
+        } else {
+          Uc = Ug[c];
+          Ac = interp ? Ag[c] : T(0);
+          Bc = interp ? Bg[c] : T(0);
+          for (int k = 0; k < n_constr; ++k) {
+            const T* Fa = static_cast<const T*>(D->Fa[k]);
+            const T* Fb = static_cast<const T*>(D->Fb[k]);
+            const int64_t ia = int64_t(sd) * D->Fa_ssd[k] + int64_t(wr) * D->Fa_sw[k] + int64_t(d) * n_c + c;
+            const int64_t ib = int64_t(sd) * D->Fb_ssd[k] + int64_t(wr) * D->Fb_sw[k] + int64_t(d) * n_c + c;
+            const T v = Fa[ia] * w + Fb[ib];
+            feas = feas && (((strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
+          }
+        }
+        if (__ballot(feas) == 0) continue;  // no lane of this wave can take action c
+        T q;
+        if (interp) {
+          const T x = Ac * w + Bc;
+          const T v = interp_row(ev, g_n, coordinate(g, x), finite_row);
+          q = Uc + beta * (cs + v);
+        } else if (terminal) {
+          q = Uc;
+        } else {
+          q = Uc + beta * cs;
+        }
+        if (feas) {
+          if (q != q) {
+            bb.nan = true;
+          } else if (!bb.seen || q > best) {
+            best = q;
+            bb.idx = d * n_c + c;
+            bb.seen = true;
+          }
         }
       }
     }
 
     if (taste) {
-      // per-discrete-action max over the continuous axis, merged over chunks
-      if (SPLIT > 1) {
-        __syncthreads();
-        red_v[tid] = best;
-        red_i[tid] = bb.idx;
-        red_f[tid] = int(bb.seen) | (int(bb.nan) << 1);
-        __syncthreads();
-        if (chunk == 0) {
-          for (int k = 1; k < SPLIT; ++k) {
-            const int j = ls + k * TILE;
-            Best o{red_i[j], (red_f[j] & 1) != 0, (red_f[j] & 2) != 0};
-            merge(best, bb, red_v[j], o);
-          }
-        }
-      }
+      // per-discrete-action max over the continuous axis, merged over groups
+      block_merge<T, TILE>(best, bb, red_v, red_i, red_f, tid, ls, grp);
       const T qd = bb.nan ? T(NAN) : (bb.seen ? best : T(-INFINITY));
       const int id = bb.seen && !bb.nan ? bb.idx : 0;
 #pragma unroll
@@ -266,7 +333,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   }
 
   if (taste) {
-    if (chunk == 0 && valid) {
+    if (grp == 0 && valid) {
       // logsum_and_softmax (logsum.py:32-64): m + scale * log(sum exp((q - m)/scale))
       const T lam = T(D->taste_scale);
       T m = -INFINITY;
@@ -295,20 +362,8 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     return;
   }
 
-  if (SPLIT > 1) {
-    red_v[tid] = best;
-    red_i[tid] = bb.idx;
-    red_f[tid] = int(bb.seen) | (int(bb.nan) << 1);
-    __syncthreads();
-    if (chunk == 0) {
-      for (int k = 1; k < SPLIT; ++k) {
-        const int j = ls + k * TILE;
-        Best o{red_i[j], (red_f[j] & 1) != 0, (red_f[j] & 2) != 0};
-        merge(best, bb, red_v[j], o);
-      }
-    }
-  }
-  if (chunk == 0 && valid) {
+  block_merge<T, TILE>(best, bb, red_v, red_i, red_f, tid, ls, grp);
+  if (grp == 0 && valid) {
     const int64_t o = int64_t(sd) * n_w + wi;
     static_cast<T*>(D->V_out)[o] = bb.nan ? T(NAN) : (bb.seen ? best : T(-INFINITY));
     D->pol_out[o] = (bb.seen && !bb.nan) ? bb.idx : 0;
@@ -338,6 +393,15 @@ int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int ti
 }
 
 }  // namespace lcm
+
+// Dynamic LDS bytes a launch needs for next-state grids of up to max_g points
+// and up to max_constr constraints (mirrors the carve in gs_period_kernel).
+extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
+  const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
+  const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
+  return 2 * g + lcm::kActChunk * 4 * sz + size_t(max_constr) * lcm::kActChunk * 2 * sz +
+         lcm::kBlock * (sz + 8) + 64;
+}
 
 extern "C" int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                                 const LcmGsTask* d_tasks, int n_tasks, int tile,
