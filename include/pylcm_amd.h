@@ -54,7 +54,9 @@ extern "C" {
  *   Q(sd,w,d,c) = U + beta * CE,  CE = sum_s ms_s[sd,d]*Vs_s
  *                                   + interp(EV[sd,d], A*w_pts[w] + B)
  *   EV[sd,d][k] = sum_r sum_j M_r[(sd*n_d + d)*n_sd_next_r + j] * Vn_r[j*g_n + k]
- *   feasible iff  Fa_k*w + Fb_k >= 0  (> 0 when bit k of strict_mask is set)
+ *   feasible iff  Fa_k*w + Fb_k >= 0  (> 0 when bit k of strict_mask is set),
+ *   or, when use_interval is set, iff  Flo <= w <= Fhi  (the same set, given
+ *   exactly when every Fa_k is -1, 0 or 1; tables (sd, d, c), stride Fi_ssd)
  * Outputs: V_out[sd*n_w + w] and pol_out[sd*n_w + w] (flat action index
  * d*n_c + c, first maximiser), or with taste shocks pol_out[(sd*n_w + w)*n_d + d]
  * (the continuous argmax per discrete action) and V the EV1 logsum.
@@ -73,13 +75,17 @@ typedef struct LcmGsDesc {
   const void* Vs[LCM_MAX_SCALAR];
   const void* ms[LCM_MAX_SCALAR];
   const void* g_pts;
+  const void* Flo;  /* with use_interval: feasible iff Flo <= w <= Fhi */
+  const void* Fhi;
   int64_t U_ssd, U_sw, A_ssd, A_sw, B_ssd, B_sw;
   int64_t Fa_ssd[LCM_MAX_CONSTR], Fa_sw[LCM_MAX_CONSTR];
   int64_t Fb_ssd[LCM_MAX_CONSTR], Fb_sw[LCM_MAX_CONSTR];
+  int64_t Fi_ssd;
   int32_t n_sd_next[LCM_MAX_TARGETS];
   int32_t n_sd, n_w, n_d, n_c;
   int32_t n_constr, strict_mask, n_targets, n_scalar;
   int32_t terminal, g_kind, g_n, has_taste;
+  int32_t use_interval, pad0;
   double beta, taste_scale, g_start, g_stop;
 } LcmGsDesc;
 
@@ -147,11 +153,13 @@ size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr);
 /* sizeof the structs above, so a binding can check its mirror of the layout. */
 int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task);
 
-/* Variant of lcm_gs_launch with an explicit choice of the kernel that reads
- * tables varying with the continuous state (wdep = 1) or not (wdep = 0). */
+/* lcm_gs_launch with an explicit kernel variant: 0 = operands staged in LDS,
+ * linear next-state grids and interval constraints (every descriptor must
+ * have use_interval set and g_kind linear); 1 = staged, any grid, constraint
+ * rows; 2 = general (tables may vary with the continuous state). */
 int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                      const LcmGsTask* d_tasks, int n_tasks, int tile,
-                     size_t lds_bytes, int wdep);
+                     size_t lds_bytes, int variant);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,40 @@ def _strides(tab: np.ndarray):
     return ssd, sw
 
 
+def interval_form(F, n_sd, n_d, n_c, dt):
+    """Constraint rows `a*w + b >= 0` (or `> 0`) with every `a` in {-1, 0, 1}
+    as one interval `lo <= w <= hi` per action, or None.
+
+    Exact in the model dtype: with `b` rounded to it, `w + b >= 0` holds iff
+    `w >= -b`, `-w + b >= 0` iff `w <= b`, and a strict bound moves to the next
+    representable number.
+    """
+    dt = np.dtype(dt)
+    n_sdx = max([1] + [max(Fa.shape[0], Fb.shape[0]) for Fa, Fb, _ in F])
+    lo = np.full((n_sdx, 1, n_d, n_c), -np.inf, dtype=dt)
+    hi = np.full((n_sdx, 1, n_d, n_c), np.inf, dtype=dt)
+    for Fa, Fb, strict in F:
+        if Fa.shape[1] != 1 or Fb.shape[1] != 1:
+            return None
+        a = np.broadcast_to(Fa, lo.shape)
+        b = np.broadcast_to(Fb, lo.shape).astype(dt)
+        if not np.all((a == 1) | (a == 0) | (a == -1)):
+            return None
+        with np.errstate(invalid="ignore"):
+            tlo = np.where(a == 1, -b, -np.inf).astype(dt)
+            thi = np.where(a == -1, b, np.inf).astype(dt)
+            if strict:
+                tlo = np.where(a == 1, np.nextafter(tlo, dt.type(np.inf)), tlo)
+                thi = np.where(a == -1, np.nextafter(thi, dt.type(-np.inf)), thi)
+            ok0 = (b > 0) if strict else (b >= 0)
+        tlo = np.where((a == 0) & ~ok0, dt.type(np.inf), tlo)
+        lo = np.maximum(lo, tlo)
+        hi = np.minimum(hi, thi)
+    if n_sdx > 1 and np.all(lo == lo[:1]) and np.all(hi == hi[:1]):
+        lo, hi = lo[:1], hi[:1]
+    return np.ascontiguousarray(lo), np.ascontiguousarray(hi)
+
+
 def default_tile(n_states_total: int) -> int:
     """States per workgroup: small workloads split actions over more waves."""
     if n_states_total <= 64 * 1024:
@@ -189,7 +223,7 @@ class GsProgram:
         self.out_end = ar.size
         descs = []
         tasks = [[] for _ in range(S)]
-        wdep = [False] * S
+        wdep = [0] * S  # kernel variant per step: the most general any descriptor needs
         gmax = [1] * S
         kmax = [0] * S
         self.flops_by_step = [0.0] * S
@@ -202,7 +236,7 @@ class GsProgram:
                 d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
                 di = len(descs)
                 descs.append(d)
-                wdep[s] |= d["wdep"]
+                wdep[s] = max(wdep[s], d["variant"])
                 kmax[s] = max(kmax[s], d["n_constr"])
                 if d["g_n"] > 0:
                     gmax[s] = max(gmax[s], d["g_n"])
@@ -318,6 +352,17 @@ class GsProgram:
                 d["B_ssd"], d["B_sw"] = _strides(pt.B)
                 wdep |= d["A_sw"] != 0 or d["B_sw"] != 0
         d["wdep"] = bool(wdep)
+        iv = interval_form(pt.F, lay.n_sd, lay.n_d, lay.n_c, dt)
+        d["use_interval"] = int(iv is not None)
+        d["Flo"] = d["Fhi"] = zero_off
+        d["Fi_ssd"] = 0
+        if iv is not None:
+            lo, hi = iv
+            d["Flo"] = ar.add(lo, dtype=dt)
+            d["Fhi"] = ar.add(hi, dtype=dt)
+            d["Fi_ssd"] = lay.n_d * lay.n_c if lo.shape[0] > 1 else 0
+        lin = (not d["g_n"]) or d["g_kind"] == 0
+        d["variant"] = 2 if wdep else (0 if (iv is not None and lin) else 1)
         return d
 
     def _desc_records(self, descs):
@@ -325,11 +370,11 @@ class GsProgram:
         base = self.base
         n = len(descs)
         D = np.zeros(n, dtype=DESC_DTYPE)
-        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts"):
+        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi"):
             D[f] = base + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
         for f in ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
                   "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
-                  "beta", "taste_scale", "g_start", "g_stop"):
+                  "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "use_interval"):
             D[f] = np.array([d[f] for d in descs], dtype=D.dtype[f])
         D["n_targets"] = np.array([len(d["Vn"]) for d in descs], dtype=np.int32)
         D["n_scalar"] = np.array([len(d["Vs"]) for d in descs], dtype=np.int32)

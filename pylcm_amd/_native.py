@@ -42,17 +42,21 @@ class LcmGsDesc(C.Structure):
         ("Vs", vp * MAX_SCALAR),
         ("ms", vp * MAX_SCALAR),
         ("g_pts", vp),
+        ("Flo", vp),
+        ("Fhi", vp),
         ("U_ssd", C.c_int64), ("U_sw", C.c_int64),
         ("A_ssd", C.c_int64), ("A_sw", C.c_int64),
         ("B_ssd", C.c_int64), ("B_sw", C.c_int64),
         ("Fa_ssd", C.c_int64 * MAX_CONSTR), ("Fa_sw", C.c_int64 * MAX_CONSTR),
         ("Fb_ssd", C.c_int64 * MAX_CONSTR), ("Fb_sw", C.c_int64 * MAX_CONSTR),
+        ("Fi_ssd", C.c_int64),
         ("n_sd_next", C.c_int32 * MAX_TARGETS),
         ("n_sd", C.c_int32), ("n_w", C.c_int32), ("n_d", C.c_int32), ("n_c", C.c_int32),
         ("n_constr", C.c_int32), ("strict_mask", C.c_int32),
         ("n_targets", C.c_int32), ("n_scalar", C.c_int32),
         ("terminal", C.c_int32), ("g_kind", C.c_int32), ("g_n", C.c_int32),
         ("has_taste", C.c_int32),
+        ("use_interval", C.c_int32), ("pad0", C.c_int32),
         ("beta", C.c_double), ("taste_scale", C.c_double),
         ("g_start", C.c_double), ("g_stop", C.c_double),
     ]

@@ -18,11 +18,14 @@
 // at the end (ties keep the smaller flat index, so the result is the first
 // argmax whatever the interleave). Per discrete action the block stages
 //   * the contracted EV row (next-period V rows weighted by M), and
-//   * chunks of CH actions' operands {A, B, U} and constraint rows {Fa, Fb}
-// in LDS. The action loop index is wave-uniform, so every per-action operand is
-// an LDS broadcast read; the only per-lane LDS traffic is the two-corner gather
-// from the EV row (one ds_read2). Blocks are remapped so that consecutive tasks
-// (same problem, same V rows) land on the same XCD and share its L2.
+//   * chunks of kActChunk actions' operands {A, B, U, lo} (+ hi, or affine
+//     constraint rows) in LDS.
+// The action loop index is wave-uniform, so every per-action operand is an LDS
+// broadcast read; the only per-lane LDS traffic is the two-corner gather from
+// the EV row (one ds_read2). Block-uniform conditions (continuation kind,
+// finite EV row) select a specialised action loop outside the loop. Blocks
+// are remapped so that consecutive tasks (same problem, same V rows) land on
+// the same XCD and share its L2.
 
 #include <type_traits>
 
@@ -34,6 +37,20 @@ constexpr int kBlock = 256;
 constexpr int kXcds = 8;
 constexpr int kActChunk = 256;  // actions staged in LDS at a time
 
+// Kernel variants, chosen per launch by the host:
+//   kVarFast:    operands staged in LDS, linear next-state grid, constraints
+//                in interval form Flo <= w <= Fhi;
+//   kVarStaged:  operands staged in LDS, any grid kind, affine constraint rows;
+//   kVarWdep:    some table varies with the continuous state: per-lane reads.
+constexpr int kVarFast = 0;
+constexpr int kVarStaged = 1;
+constexpr int kVarWdep = 2;
+
+// Continuation kinds of a descriptor (block-uniform).
+constexpr int kModeInterp = 0;    // stateful targets: interpolate EV
+constexpr int kModeTerminal = 1;  // terminal regime: Q = U
+constexpr int kModeScalar = 2;    // only stateless targets: Q = U + beta * cs
+
 template <typename T>
 struct Grid1D {
   int kind;
@@ -43,14 +60,19 @@ struct Grid1D {
 };
 
 template <typename T>
+__device__ __forceinline__ T lin_coordinate(const Grid1D<T>& g, T x) {
+  // get_linspace_coordinate (coordinates.py:30-44)
+  if constexpr (std::is_same<T, double>::value) {
+    return (x - g.start) / g.step;
+  } else {
+    return (x - g.start) * g.inv_step;
+  }
+}
+
+template <typename T>
 __device__ __forceinline__ T coordinate(const Grid1D<T>& g, T x) {
   if (g.kind == LCM_GRID_LIN) {
-    // get_linspace_coordinate (coordinates.py:30-44)
-    if constexpr (std::is_same<T, double>::value) {
-      return (x - g.start) / g.step;
-    } else {
-      return (x - g.start) * g.inv_step;
-    }
+    return lin_coordinate(g, x);
   } else if (g.kind == LCM_GRID_LOG) {
     // get_logspace_coordinate (coordinates.py:73-114)
     const T cl = (log(x) - g.log_start) / g.log_step;
@@ -72,43 +94,59 @@ __device__ __forceinline__ T coordinate(const Grid1D<T>& g, T x) {
   }
 }
 
+__device__ __forceinline__ float clamp_index(float x, float hi) {
+  return __builtin_amdgcn_fmed3f(x, 0.0f, hi);  // NaN -> in range; t stays NaN
+}
+__device__ __forceinline__ double clamp_index(double x, double hi) {
+  return fmin(fmax(x, 0.0), hi);
+}
+
 // Linear interpolation with linear extrapolation: lower index clipped to
-// [0, n-2] (ndimage.py:102-109); a zero corner weight drops its term.
-template <typename T>
-__device__ __forceinline__ T interp_row(const T* row, int n, T coord, bool finite_row) {
+// [0, n-2] (ndimage.py:102-109); with FINITE false a zero corner weight drops
+// its term (the row holds -inf somewhere).
+template <typename T, bool FINITE>
+__device__ __forceinline__ T interp_row(const T* row, int n, T coord) {
   if (n < 2) return row[0];
-  T fl = floor(coord);
-  fl = fmin(fmax(fl, T(0)), T(n - 2));
+  const T fl = clamp_index(floor(coord), T(n - 2));
   const int lo = static_cast<int>(fl);
   const T t = coord - fl;
   const T w0 = T(1) - t;
   const T v0 = row[lo];
   const T v1 = row[lo + 1];
-  if (finite_row) return w0 * v0 + t * v1;
+  if constexpr (FINITE) return w0 * v0 + t * v1;
   return zero_safe(w0, v0) + zero_safe(t, v1);
 }
 
-struct Best {
+// Running first-argmax of feasible Q: idx < 0 means no feasible action seen;
+// a feasible NaN poisons the max (jnp.max propagates NaN).
+template <typename T>
+struct Running {
+  T best;
   int idx;
-  bool seen;
   bool nan;
 };
 
-// Merge candidate (vk, k) into (vb, b): a feasible NaN poisons the max; among
-// feasible values the larger wins and ties keep the smaller flat index.
 template <typename T>
-__device__ __forceinline__ void merge(T& vb, Best& b, T vk, const Best& k) {
-  b.nan |= k.nan;
-  if (k.seen && (!b.seen || vk > vb || (vk == vb && k.idx < b.idx))) {
-    vb = vk;
-    b.idx = k.idx;
-    b.seen = true;
+__device__ __forceinline__ void offer(Running<T>& r, T q, bool feas, int a) {
+  r.nan |= feas & (q != q);
+  const bool take = feas & (q == q) & ((q > r.best) | (r.idx < 0));
+  r.best = take ? q : r.best;
+  r.idx = take ? a : r.idx;
+}
+
+// Merge candidate k into r: the larger value wins, ties keep the smaller index.
+template <typename T>
+__device__ __forceinline__ void merge(Running<T>& r, const Running<T>& k) {
+  r.nan |= k.nan;
+  if (k.idx >= 0 && (r.idx < 0 || k.best > r.best || (k.best == r.best && k.idx < r.idx))) {
+    r.best = k.best;
+    r.idx = k.idx;
   }
 }
 
 template <typename T>
 struct alignas(sizeof(T) * 4) Act {
-  T A, B, U, pad;
+  T A, B, U, lo;  // lo: lower feasibility bound on w (fast variant)
 };
 
 template <typename T>
@@ -118,31 +156,120 @@ struct alignas(sizeof(T) * 2) Con {
 
 __device__ __forceinline__ size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-// Merge the SPLIT partial results of each state into the chunk-0 thread.
+// Merge the SPLIT partial results of each state into the group-0 thread.
 template <typename T, int TILE>
-__device__ __forceinline__ void block_merge(T& best, Best& bb, T* red_v, int* red_i, int* red_f,
+__device__ __forceinline__ void block_merge(Running<T>& r, T* red_v, int* red_i, int* red_f,
                                             int tid, int ls, int grp) {
   constexpr int SPLIT = kBlock / TILE;
   if constexpr (SPLIT > 1) {
     __syncthreads();
-    red_v[tid] = best;
-    red_i[tid] = bb.idx;
-    red_f[tid] = int(bb.seen) | (int(bb.nan) << 1);
+    red_v[tid] = r.best;
+    red_i[tid] = r.idx;
+    red_f[tid] = int(r.nan);
     __syncthreads();
     if (grp == 0) {
       for (int k = 1; k < SPLIT; ++k) {
         const int j = ls + k * TILE;
-        Best o{red_i[j], (red_f[j] & 1) != 0, (red_f[j] & 2) != 0};
-        merge(best, bb, red_v[j], o);
+        merge(r, Running<T>{red_v[j], red_i[j], red_f[j] != 0});
       }
     }
   }
 }
 
-template <typename T, int TILE, bool WDEP>
+// Everything the action loop reads (block-uniform except w / valid / wr).
+template <typename T>
+struct LoopArgs {
+  const Act<T>* act;
+  const T* his;
+  const Con<T>* con;
+  const T* ev;
+  const T* Ug;
+  const T* Ag;
+  const T* Bg;
+  const LcmGsDesc* D;
+  Grid1D<T> g;
+  T w, beta, cs;
+  int n_constr, strict_mask, g_n, n_c, d, sd, wr, c0, cn, grp;
+  bool valid;
+};
+
+// The action loop over one staged chunk, specialised on the kernel variant,
+// the continuation kind and whether the EV row is finite.
+template <typename T, int VAR, int MODE, bool FINITE, int SPLIT>
+__device__ __forceinline__ void action_loop(const LoopArgs<T>& L, Running<T>& run) {
+  const T w = L.w;
+#pragma unroll 2
+  for (int i = L.grp; i < L.cn; i += SPLIT) {
+    const int c = L.c0 + i;
+    T Uc, Ac, Bc;
+    bool feas = L.valid;
+    if constexpr (VAR == kVarFast) {
+      const Act<T> a = L.act[i];
+      const T hi = L.his[i];
+      Uc = a.U;
+      Ac = a.A;
+      Bc = a.B;
+      feas = feas & (w >= a.lo) & (w <= hi);
+    } else if constexpr (VAR == kVarStaged) {
+      const Act<T> a = L.act[i];
+      Uc = a.U;
+      Ac = a.A;
+      Bc = a.B;
+      for (int k = 0; k < L.n_constr; ++k) {
+        const Con<T> f = L.con[k * kActChunk + i];
+        const T v = f.a * w + f.b;
+        feas = feas & (((L.strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
+      }
+    } else {
+      const LcmGsDesc* D = L.D;
+      Uc = L.Ug[c];
+      Ac = MODE == kModeInterp ? L.Ag[c] : T(0);
+      Bc = MODE == kModeInterp ? L.Bg[c] : T(0);
+      for (int k = 0; k < L.n_constr; ++k) {
+        const T* Fa = static_cast<const T*>(D->Fa[k]);
+        const T* Fb = static_cast<const T*>(D->Fb[k]);
+        const int64_t ia = int64_t(L.sd) * D->Fa_ssd[k] + int64_t(L.wr) * D->Fa_sw[k] + int64_t(L.d) * L.n_c + c;
+        const int64_t ib = int64_t(L.sd) * D->Fb_ssd[k] + int64_t(L.wr) * D->Fb_sw[k] + int64_t(L.d) * L.n_c + c;
+        const T v = Fa[ia] * w + Fb[ib];
+        feas = feas & (((L.strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
+      }
+    }
+    if (__ballot(feas) == 0) continue;  // no lane of this wave can take action c
+    T q;
+    if constexpr (MODE == kModeInterp) {
+      const T x = Ac * w + Bc;
+      T coord;
+      if constexpr (VAR == kVarFast) coord = lin_coordinate(L.g, x);
+      else coord = coordinate(L.g, x);
+      q = Uc + L.beta * (L.cs + interp_row<T, FINITE>(L.ev, L.g_n, coord));
+    } else if constexpr (MODE == kModeTerminal) {
+      q = Uc;
+    } else {
+      q = Uc + L.beta * L.cs;
+    }
+    offer(run, q, feas, L.d * L.n_c + c);
+  }
+}
+
+template <typename T, int VAR, int SPLIT>
+__device__ __forceinline__ void dispatch_loop(const LoopArgs<T>& L, int mode, bool finite_row,
+                                              Running<T>& run) {
+  if (mode == kModeInterp) {
+    if (finite_row) action_loop<T, VAR, kModeInterp, true, SPLIT>(L, run);
+    else action_loop<T, VAR, kModeInterp, false, SPLIT>(L, run);
+  } else if (mode == kModeTerminal) {
+    action_loop<T, VAR, kModeTerminal, true, SPLIT>(L, run);
+  } else {
+    action_loop<T, VAR, kModeScalar, true, SPLIT>(L, run);
+  }
+}
+
+template <typename T, int TILE, int VAR>
 __global__ void __launch_bounds__(kBlock)
 gs_period_kernel(const LcmGsDesc* __restrict__ descs,
                  const LcmGsTask* __restrict__ tasks, int n_tasks) {
+  constexpr bool WDEP = VAR == kVarWdep;
+  constexpr bool FAST = VAR == kVarFast;
   constexpr int SPLIT = kBlock / TILE;
   constexpr int CH = kActChunk;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -169,22 +296,40 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const bool taste = D->has_taste != 0;
   // a stateful continuation needs a next-state grid with at least one point
   const bool interp = !terminal && D->n_targets > 0 && D->g_n > 0;
+  const int mode = interp ? kModeInterp : (terminal ? kModeTerminal : kModeScalar);
   const int g_n = interp ? D->g_n : 0;
   const int n_constr = D->n_constr;
-  const int strict_mask = D->strict_mask;
 
   // LDS carve (the host sizes the launch for the largest g_n and n_constr)
   const size_t g_bytes = align16(size_t(g_n > 0 ? g_n : 1) * sizeof(T));
   T* ev = reinterpret_cast<T*>(smem);
   T* gp = reinterpret_cast<T*>(smem + g_bytes);
   Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
-  Con<T>* con = reinterpret_cast<Con<T>*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
-  T* red_v = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>) +
-                                  size_t(n_constr) * CH * sizeof(Con<T>));
+  T* his = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
+  const size_t con_ofs = 2 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
+  Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);
+  T* red_v = reinterpret_cast<T*>(smem + con_ofs + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
   int* red_f = red_i + kBlock;
 
-  Grid1D<T> g;
+  LoopArgs<T> L;
+  L.act = act;
+  L.his = his;
+  L.con = con;
+  L.ev = ev;
+  L.D = D;
+  L.w = w;
+  L.valid = valid;
+  L.n_constr = n_constr;
+  L.strict_mask = D->strict_mask;
+  L.g_n = g_n;
+  L.n_c = n_c;
+  L.sd = sd;
+  L.wr = wr;
+  L.grp = grp;
+  L.beta = T(D->beta);
+
+  Grid1D<T>& g = L.g;
   g.kind = D->g_kind;
   g.n = g_n;
   g.start = T(D->g_start);
@@ -201,13 +346,11 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     for (int k = tid; k < g_n; k += kBlock) gp[k] = src[k];
   }
 
-  const T beta = T(D->beta);
   const int64_t wofs_u = WDEP ? int64_t(wr) * D->U_sw : 0;
   const int64_t wofs_a = WDEP ? int64_t(wr) * D->A_sw : 0;
   const int64_t wofs_b = WDEP ? int64_t(wr) * D->B_sw : 0;
 
-  T best = -INFINITY;
-  Best bb{0, false, false};
+  Running<T> run{T(-INFINITY), -1, false};
   T qc[LCM_MAX_TASTE_D];
   int qci[LCM_MAX_TASTE_D];
 #pragma unroll
@@ -241,94 +384,54 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         cs += zero_safe(m, v);
       }
     }
-
-    const T* __restrict__ Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
-    const T* __restrict__ Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
-    const T* __restrict__ Bg = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
+    L.cs = cs;
+    L.d = d;
+    L.Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
+    L.Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
+    L.Bg = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
 
     for (int c0 = 0; c0 < n_c; c0 += CH) {
       const int cn = min(CH, n_c - c0);
+      L.c0 = c0;
+      L.cn = cn;
       if constexpr (!WDEP) {
         __syncthreads();  // previous chunk consumed
+        const int64_t fi = int64_t(sd) * D->Fi_ssd + int64_t(d) * n_c + c0;
         for (int i = tid; i < cn; i += kBlock) {
           Act<T> a;
-          a.U = Ug[c0 + i];
-          a.A = interp ? Ag[c0 + i] : T(0);
-          a.B = interp ? Bg[c0 + i] : T(0);
-          a.pad = T(0);
+          a.U = L.Ug[c0 + i];
+          a.A = interp ? L.Ag[c0 + i] : T(0);
+          a.B = interp ? L.Bg[c0 + i] : T(0);
+          a.lo = FAST ? static_cast<const T*>(D->Flo)[fi + i] : T(0);
           act[i] = a;
-          for (int k = 0; k < n_constr; ++k) {
-            const T* Fa = static_cast<const T*>(D->Fa[k]) + int64_t(sd) * D->Fa_ssd[k] + int64_t(d) * n_c;
-            const T* Fb = static_cast<const T*>(D->Fb[k]) + int64_t(sd) * D->Fb_ssd[k] + int64_t(d) * n_c;
-            Con<T> f;
-            f.a = Fa[c0 + i];
-            f.b = Fb[c0 + i];
-            con[k * CH + i] = f;
+          if constexpr (FAST) {
+            his[i] = static_cast<const T*>(D->Fhi)[fi + i];
+          } else {
+            for (int k = 0; k < n_constr; ++k) {
+              const T* Fa = static_cast<const T*>(D->Fa[k]) + int64_t(sd) * D->Fa_ssd[k] + int64_t(d) * n_c;
+              const T* Fb = static_cast<const T*>(D->Fb[k]) + int64_t(sd) * D->Fb_ssd[k] + int64_t(d) * n_c;
+              Con<T> f;
+              f.a = Fa[c0 + i];
+              f.b = Fb[c0 + i];
+              con[k * CH + i] = f;
+            }
           }
         }
         __syncthreads();
       }
-      for (int i = grp; i < cn; i += SPLIT) {
-        const int c = c0 + i;
-        T Uc, Ac, Bc;
-        bool feas = valid;
-        if constexpr (!WDEP) {
-          const Act<T> a = act[i];
-          Uc = a.U;
-          Ac = a.A;
-          Bc = a.B;
-          for (int k = 0; k < n_constr; ++k) {
-            const Con<T> f = con[k * CH + i];
-            const T v = f.a * w + f.b;
-            feas = feas && (((strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
-          }
-        } else {
-          Uc = Ug[c];
-          Ac = interp ? Ag[c] : T(0);
-          Bc = interp ? Bg[c] : T(0);
-          for (int k = 0; k < n_constr; ++k) {
-            const T* Fa = static_cast<const T*>(D->Fa[k]);
-            const T* Fb = static_cast<const T*>(D->Fb[k]);
-            const int64_t ia = int64_t(sd) * D->Fa_ssd[k] + int64_t(wr) * D->Fa_sw[k] + int64_t(d) * n_c + c;
-            const int64_t ib = int64_t(sd) * D->Fb_ssd[k] + int64_t(wr) * D->Fb_sw[k] + int64_t(d) * n_c + c;
-            const T v = Fa[ia] * w + Fb[ib];
-            feas = feas && (((strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
-          }
-        }
-        if (__ballot(feas) == 0) continue;  // no lane of this wave can take action c
-        T q;
-        if (interp) {
-          const T x = Ac * w + Bc;
-          const T v = interp_row(ev, g_n, coordinate(g, x), finite_row);
-          q = Uc + beta * (cs + v);
-        } else if (terminal) {
-          q = Uc;
-        } else {
-          q = Uc + beta * cs;
-        }
-        if (feas) {
-          if (q != q) {
-            bb.nan = true;
-          } else if (!bb.seen || q > best) {
-            best = q;
-            bb.idx = d * n_c + c;
-            bb.seen = true;
-          }
-        }
-      }
+      dispatch_loop<T, VAR, SPLIT>(L, mode, finite_row, run);
     }
 
     if (taste) {
       // per-discrete-action max over the continuous axis, merged over groups
-      block_merge<T, TILE>(best, bb, red_v, red_i, red_f, tid, ls, grp);
-      const T qd = bb.nan ? T(NAN) : (bb.seen ? best : T(-INFINITY));
-      const int id = bb.seen && !bb.nan ? bb.idx : 0;
+      block_merge<T, TILE>(run, red_v, red_i, red_f, tid, ls, grp);
+      const T qd = run.nan ? T(NAN) : (run.idx >= 0 ? run.best : T(-INFINITY));
+      const int id = (run.idx >= 0 && !run.nan) ? run.idx : 0;
 #pragma unroll
       for (int i = 0; i < LCM_MAX_TASTE_D; ++i) {
         if (i == d) { qc[i] = qd; qci[i] = id; }
       }
-      best = -INFINITY;
-      bb = Best{0, false, false};
+      run = Running<T>{T(-INFINITY), -1, false};
     }
   }
 
@@ -362,17 +465,19 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     return;
   }
 
-  block_merge<T, TILE>(best, bb, red_v, red_i, red_f, tid, ls, grp);
+  block_merge<T, TILE>(run, red_v, red_i, red_f, tid, ls, grp);
   if (grp == 0 && valid) {
     const int64_t o = int64_t(sd) * n_w + wi;
-    static_cast<T*>(D->V_out)[o] = bb.nan ? T(NAN) : (bb.seen ? best : T(-INFINITY));
-    D->pol_out[o] = (bb.seen && !bb.nan) ? bb.idx : 0;
+    static_cast<T*>(D->V_out)[o] = run.nan ? T(NAN) : (run.idx >= 0 ? run.best : T(-INFINITY));
+    D->pol_out[o] = (run.idx >= 0 && !run.nan) ? run.idx : 0;
   }
 }
 
 template <typename T, int TILE>
-int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, size_t lds, bool wdep) {
-  auto kern = wdep ? gs_period_kernel<T, TILE, true> : gs_period_kernel<T, TILE, false>;
+int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, size_t lds, int var) {
+  auto kern = var == kVarFast     ? gs_period_kernel<T, TILE, kVarFast>
+              : var == kVarStaged ? gs_period_kernel<T, TILE, kVarStaged>
+                                  : gs_period_kernel<T, TILE, kVarWdep>;
   if (lds > 64 * 1024) {
     LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
@@ -383,11 +488,11 @@ int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, siz
 }
 
 template <typename T>
-int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int tile, size_t lds, bool wdep) {
+int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int tile, size_t lds, int var) {
   switch (tile) {
-    case 64: return launch_tile<T, 64>(ctx, d, t, n, lds, wdep);
-    case 128: return launch_tile<T, 128>(ctx, d, t, n, lds, wdep);
-    case 256: return launch_tile<T, 256>(ctx, d, t, n, lds, wdep);
+    case 64: return launch_tile<T, 64>(ctx, d, t, n, lds, var);
+    case 128: return launch_tile<T, 128>(ctx, d, t, n, lds, var);
+    case 256: return launch_tile<T, 256>(ctx, d, t, n, lds, var);
     default: return fail(LCM_E_ARG, "tile must be 64, 128 or 256");
   }
 }
@@ -395,29 +500,32 @@ int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int ti
 }  // namespace lcm
 
 // Dynamic LDS bytes a launch needs for next-state grids of up to max_g points
-// and up to max_constr constraints (mirrors the carve in gs_period_kernel).
+// and up to max_constr constraint rows (mirrors the carve in gs_period_kernel).
 extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
-  return 2 * g + lcm::kActChunk * 4 * sz + size_t(max_constr) * lcm::kActChunk * 2 * sz +
+  const size_t his = (lcm::kActChunk * sz + 15) & ~size_t(15);
+  return 2 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
          lcm::kBlock * (sz + 8) + 64;
 }
 
 extern "C" int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                                 const LcmGsTask* d_tasks, int n_tasks, int tile,
-                                size_t lds_bytes, int wdep) {
+                                size_t lds_bytes, int variant) {
   using namespace lcm;
   if (!ctx || !d_descs || !d_tasks) return fail(LCM_E_ARG, "lcm_gs_launch: null argument");
   if (n_tasks <= 0 || n_tasks % kXcds != 0)
     return fail(LCM_E_ARG, "lcm_gs_launch: n_tasks must be a positive multiple of 8");
   if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_gs_launch: LDS request above 160 KiB");
-  if (dtype == LCM_DTYPE_F32) return launch_t<float>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, wdep != 0);
-  if (dtype == LCM_DTYPE_F64) return launch_t<double>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, wdep != 0);
+  if (variant < kVarFast || variant > kVarWdep) return fail(LCM_E_ARG, "lcm_gs_launch: unknown variant");
+  if (dtype == LCM_DTYPE_F32) return launch_t<float>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant);
+  if (dtype == LCM_DTYPE_F64) return launch_t<double>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant);
   return fail(LCM_E_ARG, "lcm_gs_launch: unknown dtype");
 }
 
+// The general variant is correct for every descriptor.
 extern "C" int lcm_gs_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                              const LcmGsTask* d_tasks, int n_tasks, int tile,
                              size_t lds_bytes) {
-  return lcm_gs_launch_ex(ctx, dtype, d_descs, d_tasks, n_tasks, tile, lds_bytes, 1);
+  return lcm_gs_launch_ex(ctx, dtype, d_descs, d_tasks, n_tasks, tile, lds_bytes, lcm::kVarWdep);
 }

@@ -1,0 +1,171 @@
+"""Test models that exercise more of the specification than the examples.
+
+`processes_model` follows the reference's tests/test_models/processes.py:44-120
+(wealth, a stochastic income process and a Markov health state; utility reads
+health; next wealth can leave the grid so the continuation extrapolates).
+"""
+
+import numpy as np
+
+from pylcm_amd import (
+    AgeGrid,
+    DiscreteGrid,
+    IrregSpacedGrid,
+    LinSpacedGrid,
+    LogNormalIIDProcess,
+    LogSpacedGrid,
+    MarkovTransition,
+    Model,
+    NormalIIDProcess,
+    Regime,
+    RouwenhorstAR1Process,
+    TauchenAR1Process,
+    UniformIIDProcess,
+    categorical,
+)
+
+
+@categorical(ordered=True)
+class Health:
+    bad: int
+    good: int
+
+
+@categorical(ordered=False)
+class AliveDead:
+    alive: int
+    dead: int
+
+
+def next_health(health, probs_array):
+    return probs_array[health]
+
+
+def pm_next_wealth(consumption, wealth):
+    return wealth - consumption
+
+
+def pm_next_regime(age, final_age_alive):
+    return np.where(age >= final_age_alive, AliveDead.dead, AliveDead.alive)
+
+
+def pm_wealth_constraint(wealth, income, consumption):
+    return wealth - consumption + np.exp(income) >= 0
+
+
+def pm_utility(wealth, income, health, consumption):
+    return np.log(consumption) * (1.0 - (1.0 - health) * 0.3)
+
+
+PROCESSES = {
+    "uniform": lambda: UniformIIDProcess(n_points=5),
+    "normal": lambda: NormalIIDProcess(n_points=5, gauss_hermite=True),
+    "lognormal": lambda: LogNormalIIDProcess(n_points=5, gauss_hermite=True),
+    "tauchen": lambda: TauchenAR1Process(n_points=5, gauss_hermite=True),
+    "rouwenhorst": lambda: RouwenhorstAR1Process(n_points=5),
+}
+
+PROCESS_PARAMS = {
+    "uniform": {"start": 0.0, "stop": 1.0},
+    "normal": {"mu": 0.0, "sigma": 1.0},
+    "lognormal": {"mu": 0.0, "sigma": 1.0},
+    "tauchen": {"rho": 0.9, "sigma": 1.0, "mu": 0.0},
+    "rouwenhorst": {"rho": 0.9, "sigma": 1.0, "mu": 0.0},
+}
+
+
+def processes_model(n_periods, distribution, wealth_grid=None):
+    final_age_alive = n_periods - 2
+    alive = Regime(
+        active=lambda age, n=final_age_alive: age <= n,
+        states={
+            "wealth": wealth_grid or LinSpacedGrid(start=1, stop=5, n_points=5),
+            "income": PROCESSES[distribution](),
+            "health": DiscreteGrid(Health),
+        },
+        state_transitions={"wealth": pm_next_wealth, "health": MarkovTransition(next_health)},
+        actions={"consumption": LinSpacedGrid(start=0.1, stop=2, n_points=4)},
+        transition=pm_next_regime,
+        constraints={"wealth_constraint": pm_wealth_constraint},
+        functions={"utility": pm_utility},
+    )
+    dead = Regime(transition=None, functions={"utility": lambda: 0.0})
+    return Model(
+        regimes={"alive": alive, "dead": dead},
+        regime_id_class=AliveDead,
+        ages=AgeGrid(start=0, stop=n_periods - 1, step="Y"),
+        fixed_params={"final_age_alive": final_age_alive},
+    )
+
+
+def processes_params(distribution):
+    return {
+        "discount_factor": 0.95,
+        "alive": {
+            "next_health": {"probs_array": np.array([[0.9, 0.1], [0.2, 0.8]])},
+            "income": dict(PROCESS_PARAMS[distribution]),
+        },
+    }
+
+
+def log_grid_model(n_periods):
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=40, n_consumption=50, income_nodes=3)
+    regs = {}
+    for n, r in m.user_regimes.items():
+        if r.terminal:
+            regs[n] = r
+            continue
+        regs[n] = r.replace(states={"wealth": LogSpacedGrid(start=1.0, stop=400.0, n_points=40),
+                                    "income": r.states["income"]})
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
+
+
+def irregular_grid_model(n_periods):
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=40, n_consumption=50)
+    pts = tuple(float(1 + 399 * (i / 39) ** 2) for i in range(40))
+    regs = {}
+    for n, r in m.user_regimes.items():
+        if r.terminal:
+            regs[n] = r
+            continue
+        regs[n] = r.replace(states={"wealth": IrregSpacedGrid(points=pts)})
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
+
+
+def taste_shock_model(n_periods):
+    from pylcm_amd import ExtremeValueTasteShocks
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=30, n_consumption=40, income_nodes=3)
+    regs = dict(m.user_regimes)
+    regs["working_life"] = regs["working_life"].replace(taste_shocks=ExtremeValueTasteShocks())
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
+
+
+def nonaffine_model(n_periods):
+    """Utility reads wealth and next wealth is nonlinear in wealth: the host
+    tabulates those over the full state grid (the general kernel variant)."""
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=30, n_consumption=40)
+
+    def u_w(consumption, wealth, is_working, disutility_of_work):
+        return np.log(consumption) + 0.01 * np.sqrt(wealth) - np.where(is_working, disutility_of_work, 0.0)
+
+    def u_r(consumption, wealth):
+        return np.log(consumption) + 0.01 * np.sqrt(wealth)
+
+    def nw(wealth, consumption, labor_income, interest_rate):
+        return (1 + interest_rate) * (wealth - consumption) + 0.1 * np.sqrt(wealth) + labor_income
+
+    regs = dict(m.user_regimes)
+    regs["working_life"] = regs["working_life"].replace(
+        functions={**regs["working_life"].functions, "utility": u_w},
+        state_transitions={"wealth": nw})
+    regs["retirement"] = regs["retirement"].replace(
+        functions={"utility": u_r}, state_transitions={"wealth": nw})
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
