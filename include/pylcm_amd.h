@@ -41,6 +41,7 @@ extern "C" {
 #define LCM_MAX_CONSTR 4
 #define LCM_MAX_SCALAR 4
 #define LCM_MAX_TASTE_D 8
+#define LCM_ACT_CHUNK 256 /* actions staged in LDS at a time */
 
 /*
  * One (problem, regime, period) of the grid-search solve. Replaces the
@@ -77,10 +78,13 @@ typedef struct LcmGsDesc {
   const void* g_pts;
   const void* Flo;  /* with use_interval: feasible iff Flo <= w <= Fhi */
   const void* Fhi;
+  /* with use_interval: per (sd, d, chunk of LCM_ACT_CHUNK actions) the pair
+   * (min Flo, max Fhi), so a block skips chunks no state of it can take */
+  const void* Fchunk;
   int64_t U_ssd, U_sw, A_ssd, A_sw, B_ssd, B_sw;
   int64_t Fa_ssd[LCM_MAX_CONSTR], Fa_sw[LCM_MAX_CONSTR];
   int64_t Fb_ssd[LCM_MAX_CONSTR], Fb_sw[LCM_MAX_CONSTR];
-  int64_t Fi_ssd;
+  int64_t Fi_ssd, Fc_ssd;
   int32_t n_sd_next[LCM_MAX_TARGETS];
   int32_t n_sd, n_w, n_d, n_c;
   int32_t n_constr, strict_mask, n_targets, n_scalar;

@@ -128,6 +128,20 @@ def interval_form(F, n_sd, n_d, n_c, dt):
     return np.ascontiguousarray(lo), np.ascontiguousarray(hi)
 
 
+def chunk_bounds(lo, hi, chunk):
+    """(min lo, max hi) per (sd, d, chunk of `chunk` actions): a block whose
+    states all lie outside [min lo, max hi] skips the chunk."""
+    n_sdx, _, n_d, n_c = lo.shape
+    n_ch = (n_c + chunk - 1) // chunk
+    pad = n_ch * chunk - n_c
+    lo_p = np.pad(lo[:, 0], ((0, 0), (0, 0), (0, pad)), constant_values=np.inf)
+    hi_p = np.pad(hi[:, 0], ((0, 0), (0, 0), (0, pad)), constant_values=-np.inf)
+    out = np.empty((n_sdx, n_d, n_ch, 2), dtype=lo.dtype)
+    out[..., 0] = lo_p.reshape(n_sdx, n_d, n_ch, chunk).min(axis=-1)
+    out[..., 1] = hi_p.reshape(n_sdx, n_d, n_ch, chunk).max(axis=-1)
+    return out
+
+
 def default_tile(n_states_total: int) -> int:
     """States per workgroup: small workloads split actions over more waves."""
     if n_states_total <= 64 * 1024:
@@ -354,13 +368,16 @@ class GsProgram:
         d["wdep"] = bool(wdep)
         iv = interval_form(pt.F, lay.n_sd, lay.n_d, lay.n_c, dt)
         d["use_interval"] = int(iv is not None)
-        d["Flo"] = d["Fhi"] = zero_off
-        d["Fi_ssd"] = 0
+        d["Flo"] = d["Fhi"] = d["Fchunk"] = zero_off
+        d["Fi_ssd"] = d["Fc_ssd"] = 0
         if iv is not None:
             lo, hi = iv
             d["Flo"] = ar.add(lo, dtype=dt)
             d["Fhi"] = ar.add(hi, dtype=dt)
             d["Fi_ssd"] = lay.n_d * lay.n_c if lo.shape[0] > 1 else 0
+            fc = chunk_bounds(lo, hi, nat.ACT_CHUNK)
+            d["Fchunk"] = ar.add(fc, dtype=dt)
+            d["Fc_ssd"] = fc.shape[1] * fc.shape[2] * 2 if fc.shape[0] > 1 else 0
         lin = (not d["g_n"]) or d["g_kind"] == 0
         d["variant"] = 2 if wdep else (0 if (iv is not None and lin) else 1)
         return d
@@ -370,11 +387,11 @@ class GsProgram:
         base = self.base
         n = len(descs)
         D = np.zeros(n, dtype=DESC_DTYPE)
-        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi"):
+        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi", "Fchunk"):
             D[f] = base + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
         for f in ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
                   "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
-                  "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "use_interval"):
+                  "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "Fc_ssd", "use_interval"):
             D[f] = np.array([d[f] for d in descs], dtype=D.dtype[f])
         D["n_targets"] = np.array([len(d["Vn"]) for d in descs], dtype=np.int32)
         D["n_scalar"] = np.array([len(d["Vs"]) for d in descs], dtype=np.int32)

@@ -23,6 +23,7 @@ MAX_TARGETS = 4
 MAX_CONSTR = 4
 MAX_SCALAR = 4
 MAX_TASTE_D = 8
+ACT_CHUNK = 256
 
 vp = C.c_void_p
 
@@ -44,12 +45,13 @@ class LcmGsDesc(C.Structure):
         ("g_pts", vp),
         ("Flo", vp),
         ("Fhi", vp),
+        ("Fchunk", vp),
         ("U_ssd", C.c_int64), ("U_sw", C.c_int64),
         ("A_ssd", C.c_int64), ("A_sw", C.c_int64),
         ("B_ssd", C.c_int64), ("B_sw", C.c_int64),
         ("Fa_ssd", C.c_int64 * MAX_CONSTR), ("Fa_sw", C.c_int64 * MAX_CONSTR),
         ("Fb_ssd", C.c_int64 * MAX_CONSTR), ("Fb_sw", C.c_int64 * MAX_CONSTR),
-        ("Fi_ssd", C.c_int64),
+        ("Fi_ssd", C.c_int64), ("Fc_ssd", C.c_int64),
         ("n_sd_next", C.c_int32 * MAX_TARGETS),
         ("n_sd", C.c_int32), ("n_w", C.c_int32), ("n_d", C.c_int32), ("n_c", C.c_int32),
         ("n_constr", C.c_int32), ("strict_mask", C.c_int32),

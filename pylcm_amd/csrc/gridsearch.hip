@@ -193,12 +193,71 @@ struct LoopArgs {
   bool valid;
 };
 
+// Feasibility of one staged action (interval or row constraints).
+template <typename T, int VAR>
+__device__ __forceinline__ bool staged_feas(const LoopArgs<T>& L, const Act<T>& a, int i) {
+  const T w = L.w;
+  if constexpr (VAR == kVarFast) {
+    return (w >= a.lo) & (w <= L.his[i]);
+  } else {
+    bool feas = true;
+    for (int k = 0; k < L.n_constr; ++k) {
+      const Con<T> f = L.con[k * kActChunk + i];
+      const T v = f.a * w + f.b;
+      feas = feas & (((L.strict_mask >> k) & 1) ? (v > T(0)) : (v >= T(0)));
+    }
+    return feas;
+  }
+}
+
+// Q of one staged action for this lane's state.
+template <typename T, int VAR, int MODE, bool FINITE>
+__device__ __forceinline__ T staged_q(const LoopArgs<T>& L, const Act<T>& a) {
+  const T w = L.w;
+  if constexpr (MODE == kModeInterp) {
+    const T x = a.A * w + a.B;
+    T coord;
+    if constexpr (VAR == kVarFast) coord = lin_coordinate(L.g, x);
+    else coord = coordinate(L.g, x);
+    return a.U + L.beta * (L.cs + interp_row<T, FINITE>(L.ev, L.g_n, coord));
+  } else if constexpr (MODE == kModeTerminal) {
+    return a.U;
+  } else {
+    return a.U + L.beta * L.cs;
+  }
+}
+
 // The action loop over one staged chunk, specialised on the kernel variant,
-// the continuation kind and whether the EV row is finite.
+// the continuation kind and whether the EV row is finite. Two actions per
+// iteration (offered in increasing order, so the first argmax is kept) give
+// the scheduler independent LDS reads to overlap.
+template <typename T, int VAR, int MODE, bool FINITE, int SPLIT>
+__device__ __forceinline__ void staged_loop(const LoopArgs<T>& L, Running<T>& run) {
+  for (int i = L.grp; i < L.cn; i += 2 * SPLIT) {
+    const int i1 = i + SPLIT;
+    const bool has1 = i1 < L.cn;
+    const int j1 = has1 ? i1 : i;  // stay inside the staged chunk
+    const Act<T> a0 = L.act[i];
+    const Act<T> a1 = L.act[j1];
+    const bool f0 = L.valid & staged_feas<T, VAR>(L, a0, i);
+    const bool f1 = L.valid & has1 & staged_feas<T, VAR>(L, a1, j1);
+    if (__ballot(f0 | f1) == 0) continue;  // no lane of this wave can take either action
+    const T q0 = staged_q<T, VAR, MODE, FINITE>(L, a0);
+    const T q1 = staged_q<T, VAR, MODE, FINITE>(L, a1);
+    offer(run, q0, f0, L.d * L.n_c + L.c0 + i);
+    offer(run, q1, f1, L.d * L.n_c + L.c0 + i1);
+  }
+}
+
+// The action loop over one chunk, specialised on the kernel variant, the
+// continuation kind and whether the EV row is finite.
 template <typename T, int VAR, int MODE, bool FINITE, int SPLIT>
 __device__ __forceinline__ void action_loop(const LoopArgs<T>& L, Running<T>& run) {
+  if constexpr (VAR != kVarWdep) {
+    staged_loop<T, VAR, MODE, FINITE, SPLIT>(L, run);
+    return;
+  }
   const T w = L.w;
-#pragma unroll 2
   for (int i = L.grp; i < L.cn; i += SPLIT) {
     const int c = L.c0 + i;
     T Uc, Ac, Bc;
@@ -292,6 +351,8 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const bool valid = wi < n_w;
   const int wr = valid ? wi : 0;
   const T w = static_cast<const T*>(D->w_pts)[wr];
+  const T w_first = static_cast<const T*>(D->w_pts)[tk.w0];
+  const T w_last = static_cast<const T*>(D->w_pts)[min(tk.w0 + TILE, n_w) - 1];
   const bool terminal = D->terminal != 0;
   const bool taste = D->has_taste != 0;
   // a stateful continuation needs a next-state grid with at least one point
@@ -394,6 +455,12 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
       const int cn = min(CH, n_c - c0);
       L.c0 = c0;
       L.cn = cn;
+      if constexpr (FAST) {
+        // skip a chunk no state of this block can take (w_pts increase)
+        const T* fc = static_cast<const T*>(D->Fchunk) + int64_t(sd) * D->Fc_ssd +
+                      (int64_t(d) * ((n_c + CH - 1) / CH) + c0 / CH) * 2;
+        if (fc[0] > w_last || fc[1] < w_first) continue;
+      }
       if constexpr (!WDEP) {
         __syncthreads();  // previous chunk consumed
         const int64_t fi = int64_t(sd) * D->Fi_ssd + int64_t(d) * n_c + c0;
