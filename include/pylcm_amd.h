@@ -101,6 +101,85 @@ typedef struct LcmGsTask {
   int32_t pad;
 } LcmGsTask;
 
+/*
+ * Standalone EV contraction of one (problem, regime, period): for every
+ * discrete state sd and discrete action d,
+ *   EV[(sd*n_d + d)*g_n + k] = sum_r sum_j M_r[(sd*n_d + d)*n_sd_next_r + j] * Vn_r[j*g_n + k]
+ *   CS[sd*n_d + d]           = sum_s ms_s[sd*n_d + d] * Vs_s        (zero-safe)
+ * reading M, Vn, ms, Vs from the grid-search descriptor gs. This is the
+ * expectation over next nodes (Q_and_F.py:1145-1209) done once per row; the
+ * forward simulation interpolates its rows at off-grid states.
+ */
+typedef struct LcmEvDesc {
+  const LcmGsDesc* gs; /* device pointer */
+  void* EV;
+  void* CS;
+  int32_t n_sd, n_d, g_n, pad;
+} LcmEvDesc;
+
+#define LCM_SIM_MAX_FEAT 4
+#define LCM_SIM_MAX_REGIMES 8
+
+/*
+ * One (regime, period) of the forward simulation: per agent, the argmax over
+ * the action grid at the agent's own (off-grid) state and the draw of its next
+ * state. Stands in for `_simulate_regime_in_period`
+ * (src/_lcm/simulation/simulate.py:465-620), `argmax_and_max_Q_over_a`
+ * (src/_lcm/regime_building/max_Q_over_a.py:194-321) and
+ * `calculate_next_states` / `_draw_random_regime_ids`
+ * (src/_lcm/simulation/transitions.py:66-339).
+ * Tables are indexed (sdd, d, c) with sdd the agent's discrete (non-process)
+ * state: element at (sdd*n_d + d)*n_c + c.
+ *   phi_0 = process value z (when n_nodes > 0); phi_k = op_k(sum_i fcoef_k[sdd*F+i]*phi_i + fconst_k[sdd])
+ *   U = U0 + sum_k Uf_k*phi_k;   w' = A*w + B0 + sum_k Bf_k*phi_k;   feasible iff lo <= w <= hi
+ *   CE = CS[row] + (1-t) interp(EV[row_lo, d], w') + t interp(EV[row_hi, d], w')
+ *   with row = sd_base[sdd] + node*p_stride and (node, t) the process coordinate of z.
+ */
+typedef struct LcmSimDesc {
+  const void* U0;
+  const void* A;
+  const void* B0;
+  const void* lo;
+  const void* hi;
+  const void* Uf[LCM_SIM_MAX_FEAT];
+  const void* Bf[LCM_SIM_MAX_FEAT];
+  const void* fcoef[LCM_SIM_MAX_FEAT];  /* [n_sdd][LCM_SIM_MAX_FEAT] */
+  const void* fconst[LCM_SIM_MAX_FEAT]; /* [n_sdd] */
+  const void* EV;        /* [n_sd_full][n_d][g_n] */
+  const void* CS;        /* [n_sd_full][n_d] */
+  const int32_t* sd_base; /* [n_sdd] */
+  const void* nodes;     /* process nodes [n_nodes] */
+  const void* g_pts;     /* next continuous grid */
+  const void* preg;      /* [n_sdd][n_d][n_regimes] regime probabilities */
+  const void* Wd[LCM_SIM_MAX_REGIMES]; /* per target regime [n_sdd][n_d][n_sdd_r] */
+  int32_t n_sdd_next[LCM_SIM_MAX_REGIMES];
+  int32_t feat_op[LCM_SIM_MAX_FEAT];
+  int32_t n_feat, n_sdd, n_d, n_c;
+  int32_t terminal, interp, n_nodes, p_stride;
+  int32_t g_kind, g_n, n_regimes, regime_code;
+  int32_t proc_kind, n_rows; /* n_rows = rows of EV (flat discrete states incl. process) */
+  double beta, g_start, g_stop, pa, pb, pc;
+} LcmSimDesc;
+
+/* Agent panel state (device arrays of n agents). */
+typedef struct LcmAgents {
+  void* w;          /* continuous state, model dtype */
+  void* z;          /* process value, model dtype */
+  int32_t* sdd;     /* discrete state (flat, within the agent's regime) */
+  int32_t* regime;  /* regime code, -1 before the agent's first period */
+  int32_t* start;   /* first period of the agent */
+} LcmAgents;
+
+/* Per-period outputs (device arrays of n agents). */
+typedef struct LcmSimOut {
+  int32_t* regime;
+  int32_t* action;  /* flat action index d*n_c + c, -1 when not simulated */
+  void* value;
+  void* w;
+  void* z;
+  int32_t* sdd;
+} LcmSimOut;
+
 typedef struct lcm_ctx lcm_ctx;
 
 /* Select and initialise the device. Replaces JAX backend initialisation. */
@@ -156,6 +235,34 @@ size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr);
 
 /* sizeof the structs above, so a binding can check its mirror of the layout. */
 int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task);
+int lcm_abi_sizes_sim(size_t* ev_desc, size_t* sim_desc, size_t* agents, size_t* sim_out);
+
+/* EV contraction for n_desc descriptors (one workgroup per (desc, sd, d)). */
+int lcm_ev_launch(lcm_ctx* ctx, int dtype, const LcmEvDesc* d_descs, int n_desc,
+                  int max_rows);
+
+/*
+ * One period of forward simulation for all regimes active in it: sim_descs
+ * holds one LcmSimDesc per regime code (n_regimes entries; a regime inactive
+ * this period has U0 == NULL). Agents whose regime is r and whose start period
+ * is <= period take the argmax over the action grid; their next state and
+ * next regime are drawn with Philox4x32-10 keyed by seed, counter
+ * (agent, period, stream, 0). cur/next are the agent panels of this and the
+ * next period; out receives this period's record. d_counts[n_regimes] and
+ * d_order[n_regimes * n_agents] are device scratch for the partition of the
+ * agents by regime. Replaces the period loop body of
+ * src/_lcm/simulation/simulate.py:296-378.
+ */
+int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                      const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
+                      int n_agents, int period, uint64_t seed, size_t lds_bytes,
+                      int32_t* d_counts, int32_t* d_order);
+size_t lcm_sim_lds_bytes(int dtype, int max_g, int max_ev_rows, int max_act);
+
+/* Raw Philox4x32-10 blocks: out[4*i .. 4*i+3] for counter (i, c1, c2, c3) and
+ * key (k0, k1); for known-answer tests of the device generator. */
+int lcm_philox(lcm_ctx* ctx, uint32_t* d_out, int n, uint32_t c1, uint32_t c2, uint32_t c3,
+               uint32_t k0, uint32_t k1);
 
 /* lcm_gs_launch with an explicit kernel variant: 0 = operands staged in LDS,
  * linear next-state grids and interval constraints (every descriptor must

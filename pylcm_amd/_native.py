@@ -68,6 +68,41 @@ class LcmGsTask(C.Structure):
     _fields_ = [("desc", C.c_int32), ("sd", C.c_int32), ("w0", C.c_int32), ("pad", C.c_int32)]
 
 
+SIM_MAX_FEAT = 4
+SIM_MAX_REGIMES = 8
+
+
+class LcmEvDesc(C.Structure):
+    _fields_ = [("gs", vp), ("EV", vp), ("CS", vp),
+                ("n_sd", C.c_int32), ("n_d", C.c_int32), ("g_n", C.c_int32), ("pad", C.c_int32)]
+
+
+class LcmSimDesc(C.Structure):
+    _fields_ = [
+        ("U0", vp), ("A", vp), ("B0", vp), ("lo", vp), ("hi", vp),
+        ("Uf", vp * SIM_MAX_FEAT), ("Bf", vp * SIM_MAX_FEAT),
+        ("fcoef", vp * SIM_MAX_FEAT), ("fconst", vp * SIM_MAX_FEAT),
+        ("EV", vp), ("CS", vp), ("sd_base", vp), ("nodes", vp), ("g_pts", vp), ("preg", vp),
+        ("Wd", vp * SIM_MAX_REGIMES),
+        ("n_sdd_next", C.c_int32 * SIM_MAX_REGIMES),
+        ("feat_op", C.c_int32 * SIM_MAX_FEAT),
+        ("n_feat", C.c_int32), ("n_sdd", C.c_int32), ("n_d", C.c_int32), ("n_c", C.c_int32),
+        ("terminal", C.c_int32), ("interp", C.c_int32), ("n_nodes", C.c_int32), ("p_stride", C.c_int32),
+        ("g_kind", C.c_int32), ("g_n", C.c_int32), ("n_regimes", C.c_int32), ("regime_code", C.c_int32),
+        ("proc_kind", C.c_int32), ("n_rows", C.c_int32),
+        ("beta", C.c_double), ("g_start", C.c_double), ("g_stop", C.c_double),
+        ("pa", C.c_double), ("pb", C.c_double), ("pc", C.c_double),
+    ]
+
+
+class LcmAgents(C.Structure):
+    _fields_ = [("w", vp), ("z", vp), ("sdd", vp), ("regime", vp), ("start", vp)]
+
+
+class LcmSimOut(C.Structure):
+    _fields_ = [("regime", vp), ("action", vp), ("value", vp), ("w", vp), ("z", vp), ("sdd", vp)]
+
+
 _SIGS = {
     "lcm_init": ([C.c_int], C.c_int),
     "lcm_last_error": ([], C.c_char_p),
@@ -96,6 +131,12 @@ _SIGS = {
     "lcm_event_elapsed": ([vp, vp, C.POINTER(C.c_float)], C.c_int),
     "lcm_abi_sizes": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
     "lcm_gs_lds_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),
+    "lcm_abi_sizes_sim": ([C.POINTER(C.c_size_t)] * 4, C.c_int),
+    "lcm_ev_launch": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),
+    "lcm_sim_period_ex": ([vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_uint64,
+                           C.c_size_t, vp, vp], C.c_int),
+    "lcm_sim_lds_bytes": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
+    "lcm_philox": ([vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32], C.c_int),
 }
 
 _lock = threading.Lock()

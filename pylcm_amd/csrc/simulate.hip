@@ -1,0 +1,457 @@
+// Forward simulation kernels for gfx950: Philox4x32-10 counter-based RNG, the
+// EV contraction, the agent partition by regime and the per-agent argmax and
+// state transition. Declarations and reference citations: include/pylcm_amd.h.
+
+#include <type_traits>
+
+#include "common.h"
+
+namespace lcm {
+
+constexpr int kSimBlock = 256;
+constexpr int kSimChunk = 128;  // actions staged in LDS at a time
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplications with Weyl key
+// schedule.
+// ---------------------------------------------------------------------------
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+  constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const uint64_t p0 = uint64_t(M0) * c.x;
+    const uint64_t p1 = uint64_t(M1) * c.z;
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+  }
+  return c;
+}
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+__device__ __forceinline__ double uniform53(uint32_t a, uint32_t b) {
+  return (double(a >> 5) * 67108864.0 + double(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Box-Muller standard normal from one Philox block (first of the pair).
+__device__ __forceinline__ double normal_bm(const U4& r) {
+  const double u1 = 1.0 - uniform53(r.x, r.y);  // (0, 1]
+  const double u2 = uniform53(r.z, r.w);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+constexpr uint32_t kStreamRegime = 0;
+constexpr uint32_t kStreamProcess = 1;
+constexpr uint32_t kStreamDiscrete = 2;
+
+__global__ void philox_kernel(uint32_t* out, int n, uint32_t c1, uint32_t c2, uint32_t c3,
+                              uint32_t k0, uint32_t k1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const U4 r = philox4x32_10(U4{uint32_t(i), c1, c2, c3}, k0, k1);
+  out[4 * i + 0] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+// ---------------------------------------------------------------------------
+// EV contraction: one workgroup per (descriptor, sd, d) row.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256)
+ev_kernel(const LcmEvDesc* __restrict__ descs, int max_rows) {
+  const int di = blockIdx.y;
+  const LcmEvDesc E = descs[di];
+  const int row = blockIdx.x;
+  if (row >= E.n_sd * E.n_d) return;
+  const LcmGsDesc* __restrict__ D = E.gs;
+  const int n_d = E.n_d, g_n = E.g_n;
+  const int sd = row / n_d, d = row % n_d;
+  T* out = static_cast<T*>(E.EV) + int64_t(row) * g_n;
+  for (int k = threadIdx.x; k < g_n; k += blockDim.x) {
+    T acc = T(0);
+    for (int r = 0; r < D->n_targets; ++r) {
+      const int nj = D->n_sd_next[r];
+      const T* __restrict__ Mr = static_cast<const T*>(D->M[r]) + (int64_t(sd) * n_d + d) * nj;
+      const T* __restrict__ Vr = static_cast<const T*>(D->Vn[r]);
+      for (int j = 0; j < nj; ++j) {
+        const T m = Mr[j];
+        if (m != T(0)) acc += m * Vr[int64_t(j) * g_n + k];
+      }
+    }
+    out[k] = acc;
+  }
+  if (threadIdx.x == 0) {
+    T cs = T(0);
+    for (int s = 0; s < D->n_scalar; ++s) {
+      const T m = static_cast<const T*>(D->ms[s])[int64_t(sd) * n_d + d];
+      const T v = static_cast<const T*>(D->Vs[s])[0];
+      cs += zero_safe(m, v);
+    }
+    static_cast<T*>(E.CS)[row] = cs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partition agents by regime (order within a list does not affect results).
+// ---------------------------------------------------------------------------
+__global__ void partition_kernel(const int32_t* __restrict__ regime, const int32_t* __restrict__ start,
+                                 int n, int period, int n_regimes, int32_t* __restrict__ counts,
+                                 int32_t* __restrict__ order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = regime[i];
+  if (start[i] > period || r < 0 || r >= n_regimes) return;
+  const int slot = atomicAdd(&counts[r], 1);
+  order[int64_t(r) * n + slot] = i;
+}
+
+__global__ void clear_kernel(int32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void fill_kernel(int32_t* p, int n, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Per-agent argmax over actions and next-state draw.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T sim_coordinate(int kind, int n, T start, T stop, const T* pts, T x) {
+  if (n < 2) return T(0);
+  if (kind == LCM_GRID_LIN) {
+    const T step = (stop - start) / T(n - 1);
+    return (x - start) / step;
+  } else if (kind == LCM_GRID_LOG) {
+    const T ls = log(start), step = (log(stop) - ls) / T(n - 1);
+    const T cl = (log(x) - ls) / step;
+    const T rlo = floor(cl);
+    const T lower = exp(ls + step * rlo), upper = exp(ls + step * (rlo + T(1)));
+    return rlo + (x - lower) / (upper - lower);
+  }
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (pts[mid] <= x) lo = mid + 1; else hi = mid;
+  }
+  const int up = min(max(lo, 1), n - 1), dn = up - 1;
+  return T(dn) + (x - pts[dn]) / (pts[up] - pts[dn]);
+}
+
+template <typename T>
+__device__ __forceinline__ T interp_at(const T* row, int n, T coord) {
+  if (n < 2) return row[0];
+  T fl = fmin(fmax(floor(coord), T(0)), T(n - 2));
+  const int lo = int(fl);
+  const T t = coord - fl;
+  return zero_safe(T(1) - t, row[lo]) + zero_safe(t, row[lo + 1]);
+}
+
+template <typename T>
+__device__ __forceinline__ T apply_op(int op, T x) {
+  switch (op) {
+    case 1: return exp(x);
+    case 2: return log(x);
+    case 3: return sqrt(x);
+    case 4: return -x;
+    default: return x;
+  }
+}
+
+// Draw an index from categorical weights p[0..n) with uniform u (CDF walk).
+template <typename T>
+__device__ __forceinline__ int draw_categorical(const T* p, int n, double u) {
+  double total = 0.0;
+  for (int i = 0; i < n; ++i) total += double(p[i]);
+  double acc = 0.0;
+  int last = 0;
+  for (int i = 0; i < n; ++i) {
+    if (p[i] > T(0)) last = i;
+    acc += double(p[i]);
+    if (p[i] > T(0) && u * total < acc) return i;
+  }
+  return last;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kSimBlock)
+sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, LcmAgents nxt,
+           LcmSimOut out, const int32_t* __restrict__ counts, const int32_t* __restrict__ order,
+           int n_agents, int period, uint32_t k0, uint32_t k1) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int r = blockIdx.y;
+  const LcmSimDesc* __restrict__ D = descs + r;
+  if (D->U0 == nullptr) return;
+  const int cnt = counts[r];
+  const int slot = blockIdx.x * kSimBlock + threadIdx.x;
+  if (int(blockIdx.x) * kSimBlock >= cnt) return;  // whole block idle
+  const bool valid = slot < cnt;
+  const int agent = valid ? order[int64_t(r) * n_agents + slot] : 0;
+
+  const int n_sdd = D->n_sdd, n_d = D->n_d, n_c = D->n_c, nf = D->n_feat;
+  const bool interp = D->interp != 0;
+  const int g_n = interp ? D->g_n : 0;
+  const int n_nodes = D->n_nodes;
+  const int sdd = valid ? cur.sdd[agent] : 0;
+  const T w = valid ? static_cast<const T*>(cur.w)[agent] : T(0);
+  const T z = valid ? static_cast<const T*>(cur.z)[agent] : T(0);
+
+  // per-agent features
+  T phi[LCM_SIM_MAX_FEAT];
+#pragma unroll
+  for (int k = 0; k < LCM_SIM_MAX_FEAT; ++k) phi[k] = T(0);
+  for (int k = 0; k < nf; ++k) {
+    const int op = D->feat_op[k];
+    T v;
+    if (op == 0) {
+      v = z;
+    } else {
+      const T* fc = static_cast<const T*>(D->fcoef[k]) + int64_t(sdd) * LCM_SIM_MAX_FEAT;
+      v = static_cast<const T*>(D->fconst[k])[sdd];
+#pragma unroll
+      for (int i = 0; i < LCM_SIM_MAX_FEAT; ++i) v += (i < k ? fc[i] * phi[i] : T(0));
+      v = apply_op(op, v);
+    }
+#pragma unroll
+    for (int i = 0; i < LCM_SIM_MAX_FEAT; ++i) if (i == k) phi[i] = v;
+  }
+
+  // process coordinate of z on the nodes: rows row_lo, row_lo + p_stride, weight t
+  const T* nodes = static_cast<const T*>(D->nodes);
+  int row_lo = D->sd_base[sdd];  // flat discrete state of process node 0
+  T tz = T(0);
+  if (interp && n_nodes > 1) {
+    const T cz = sim_coordinate<T>(LCM_GRID_IRREG, n_nodes, T(0), T(0), nodes, z);
+    const T fl = fmin(fmax(floor(cz), T(0)), T(n_nodes - 2));
+    tz = cz - fl;
+    row_lo += int(fl) * D->p_stride;
+  }
+  const int row_hi = row_lo + (n_nodes > 1 ? D->p_stride : 0);
+  const int n_rows = interp ? D->n_rows : 0;
+
+  // LDS: EV rows of the current d [n_rows][g_n], grid points, action chunk tables
+  T* ev = reinterpret_cast<T*>(smem);
+  T* gp = ev + (n_rows * g_n > 0 ? n_rows * g_n : 1);
+  const int ntab = 5 + 2 * nf;  // U0 A B0 lo hi Uf.. Bf..
+  T* tab = gp + (g_n > 0 ? g_n : 1);
+  if (interp && D->g_kind == LCM_GRID_IRREG) {
+    for (int k = threadIdx.x; k < g_n; k += kSimBlock) gp[k] = static_cast<const T*>(D->g_pts)[k];
+  }
+  const T g_start = T(D->g_start), g_stop = T(D->g_stop);
+  const T beta = T(D->beta);
+
+  T best = -INFINITY;
+  int bidx = -1;
+  bool nanf = false;
+  for (int d = 0; d < n_d; ++d) {
+    __syncthreads();
+    if (interp) {
+      const T* EVg = static_cast<const T*>(D->EV);
+      for (int i = threadIdx.x; i < n_rows * g_n; i += kSimBlock) {
+        const int rr = i / g_n, k = i % g_n;
+        ev[i] = EVg[(int64_t(rr) * n_d + d) * g_n + k];
+      }
+    }
+    T cs = T(0);
+    if (!D->terminal) {
+      const T* CS = static_cast<const T*>(D->CS);
+      cs = CS ? CS[int64_t(row_lo) * n_d + d] : T(0);
+    }
+    for (int c0 = 0; c0 < n_c; c0 += kSimChunk) {
+      const int cn = min(kSimChunk, n_c - c0);
+      __syncthreads();
+      // stage tables for all sdd: tab[((s*ntab + j)*kSimChunk) + i]
+      for (int idx = threadIdx.x; idx < n_sdd * cn; idx += kSimBlock) {
+        const int s = idx / cn, i = idx % cn;
+        const int64_t e = (int64_t(s) * n_d + d) * n_c + c0 + i;
+        T* base = tab + int64_t(s) * ntab * kSimChunk + i;
+        base[0 * kSimChunk] = static_cast<const T*>(D->U0)[e];
+        base[1 * kSimChunk] = interp ? static_cast<const T*>(D->A)[e] : T(0);
+        base[2 * kSimChunk] = interp ? static_cast<const T*>(D->B0)[e] : T(0);
+        base[3 * kSimChunk] = static_cast<const T*>(D->lo)[e];
+        base[4 * kSimChunk] = static_cast<const T*>(D->hi)[e];
+        for (int k = 0; k < nf; ++k) {
+          base[(5 + k) * kSimChunk] = D->Uf[k] ? static_cast<const T*>(D->Uf[k])[e] : T(0);
+          base[(5 + nf + k) * kSimChunk] = (interp && D->Bf[k]) ? static_cast<const T*>(D->Bf[k])[e] : T(0);
+        }
+      }
+      __syncthreads();
+      const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
+      for (int i = 0; i < cn; ++i) {
+        const bool feas = valid && (w >= mine[3 * kSimChunk + i]) && (w <= mine[4 * kSimChunk + i]);
+        if (__ballot(feas) == 0) continue;
+        T u = mine[i];
+        for (int k = 0; k < nf; ++k) u += mine[(5 + k) * kSimChunk + i] * phi[k];
+        T q;
+        if (D->terminal) {
+          q = u;
+        } else if (interp) {
+          T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
+          for (int k = 0; k < nf; ++k) x += mine[(5 + nf + k) * kSimChunk + i] * phi[k];
+          const T coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
+          const T v_lo = interp_at(ev + int64_t(row_lo) * g_n, g_n, coord);
+          T v = zero_safe(T(1) - tz, v_lo);
+          if (n_nodes > 1) v += zero_safe(tz, interp_at(ev + int64_t(row_hi) * g_n, g_n, coord));
+          q = u + beta * (cs + v);
+        } else {
+          q = u + beta * cs;
+        }
+        if (feas) {
+          nanf |= q != q;
+          if (q == q && (bidx < 0 || q > best)) {
+            best = q;
+            bidx = d * n_c + c0 + i;
+          }
+        }
+      }
+    }
+  }
+  if (!valid) return;
+
+  // record this period
+  const int64_t o = int64_t(period) * n_agents + agent;
+  out.regime[o] = r;
+  out.action[o] = (bidx >= 0 && !nanf) ? bidx : 0;
+  static_cast<T*>(out.value)[o] = nanf ? T(NAN) : (bidx >= 0 ? best : T(-INFINITY));
+  static_cast<T*>(out.w)[o] = w;
+  static_cast<T*>(out.z)[o] = z;
+  out.sdd[o] = sdd;
+
+  // next state; a terminal regime is absorbing with frozen state
+  if (D->terminal) {
+    nxt.regime[agent] = r;
+    return;
+  }
+  const int a = bidx >= 0 ? bidx : 0;
+  const int d = a / n_c, c = a % n_c;
+  const int64_t e = (int64_t(sdd) * n_d + d) * n_c + c;
+  T x = static_cast<const T*>(D->A)[e] * w + static_cast<const T*>(D->B0)[e];
+  for (int k = 0; k < nf; ++k) x += (D->Bf[k] ? static_cast<const T*>(D->Bf[k])[e] : T(0)) * phi[k];
+
+  const U4 rr = philox4x32_10(U4{uint32_t(agent), uint32_t(period), kStreamRegime, 0u}, k0, k1);
+  const T* pr = static_cast<const T*>(D->preg) + (int64_t(sdd) * n_d + d) * D->n_regimes;
+  const int nr = draw_categorical(pr, D->n_regimes, uniform53(rr.x, rr.y));
+  int nsdd = 0;
+  if (nr < LCM_SIM_MAX_REGIMES && D->Wd[nr] != nullptr) {
+    const U4 rd = philox4x32_10(U4{uint32_t(agent), uint32_t(period), kStreamDiscrete, 0u}, k0, k1);
+    const int nn = D->n_sdd_next[nr];
+    const T* wd = static_cast<const T*>(D->Wd[nr]) + (int64_t(sdd) * n_d + d) * nn;
+    nsdd = draw_categorical(wd, nn, uniform53(rd.x, rd.y));
+  }
+  T zn = z;
+  if (n_nodes > 0) {
+    const U4 rp = philox4x32_10(U4{uint32_t(agent), uint32_t(period), kStreamProcess, 0u}, k0, k1);
+    const double nrm = normal_bm(rp);
+    const double u = uniform53(rp.z, rp.w);
+    const double pa = D->pa, pb = D->pb, pc = D->pc;
+    double v;
+    switch (D->proc_kind) {
+      case 0: v = pa + pb * double(z) + pc * nrm; break;  // AR(1)
+      case 1: v = pa + pc * nrm; break;                   // IID normal
+      case 2: v = exp(pa + pc * nrm); break;              // IID lognormal
+      default: v = pa + (pb - pa) * u; break;             // IID uniform
+    }
+    zn = T(v);
+  }
+  static_cast<T*>(nxt.w)[agent] = x;
+  static_cast<T*>(nxt.z)[agent] = zn;
+  nxt.sdd[agent] = nsdd;
+  nxt.regime[agent] = nr;
+}
+
+}  // namespace lcm
+
+using namespace lcm;
+
+extern "C" int lcm_abi_sizes_sim(size_t* ev_desc, size_t* sim_desc, size_t* agents, size_t* sim_out) {
+  if (ev_desc) *ev_desc = sizeof(LcmEvDesc);
+  if (sim_desc) *sim_desc = sizeof(LcmSimDesc);
+  if (agents) *agents = sizeof(LcmAgents);
+  if (sim_out) *sim_out = sizeof(LcmSimOut);
+  return LCM_OK;
+}
+
+extern "C" int lcm_philox(lcm_ctx* ctx, uint32_t* d_out, int n, uint32_t c1, uint32_t c2, uint32_t c3,
+                          uint32_t k0, uint32_t k1) {
+  if (!ctx || !d_out || n <= 0) return fail(LCM_E_ARG, "lcm_philox: bad argument");
+  hipLaunchKernelGGL(philox_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_out, n, c1, c2,
+                     c3, k0, k1);
+  LCM_HIP(hipGetLastError());
+  return LCM_OK;
+}
+
+extern "C" int lcm_ev_launch(lcm_ctx* ctx, int dtype, const LcmEvDesc* d_descs, int n_desc, int max_rows) {
+  if (!ctx || !d_descs || n_desc <= 0 || max_rows <= 0) return fail(LCM_E_ARG, "lcm_ev_launch: bad argument");
+  dim3 grid(max_rows, n_desc);
+  if (dtype == LCM_DTYPE_F32)
+    hipLaunchKernelGGL(ev_kernel<float>, grid, dim3(256), 0, ctx->stream, d_descs, max_rows);
+  else if (dtype == LCM_DTYPE_F64)
+    hipLaunchKernelGGL(ev_kernel<double>, grid, dim3(256), 0, ctx->stream, d_descs, max_rows);
+  else
+    return fail(LCM_E_ARG, "lcm_ev_launch: unknown dtype");
+  LCM_HIP(hipGetLastError());
+  return LCM_OK;
+}
+
+extern "C" size_t lcm_sim_lds_bytes(int dtype, int max_g, int max_ev_rows, int max_act) {
+  const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
+  const size_t ev = size_t(max_ev_rows > 0 ? max_ev_rows : 1) * size_t(max_g > 0 ? max_g : 1);
+  return (ev + size_t(max_g > 0 ? max_g : 1) + size_t(max_act) * kSimChunk) * sz + 64;
+}
+
+extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                                 const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
+                                 int n_agents, int period, uint64_t seed, size_t lds_bytes,
+                                 int32_t* d_counts, int32_t* d_order) {
+  if (!ctx || !d_sim_descs || !cur || !next || !out || !d_counts || !d_order)
+    return fail(LCM_E_ARG, "lcm_sim_period: null argument");
+  if (n_agents <= 0 || n_regimes <= 0 || n_regimes > LCM_SIM_MAX_REGIMES)
+    return fail(LCM_E_ARG, "lcm_sim_period: bad sizes");
+  if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_sim_period: LDS request above 160 KiB");
+  const int nb = (n_agents + 255) / 256;
+  hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, ctx->stream, d_counts, n_regimes);
+  hipLaunchKernelGGL(partition_kernel, dim3(nb), dim3(256), 0, ctx->stream, cur->regime, cur->start,
+                     n_agents, period, n_regimes, d_counts, d_order);
+  // agents not simulated this period keep their state; the next-period regime
+  // starts as the current one and is overwritten for simulated agents
+  LCM_HIP(hipMemcpyAsync(next->regime, cur->regime, sizeof(int32_t) * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
+  LCM_HIP(hipMemcpyAsync(next->sdd, cur->sdd, sizeof(int32_t) * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
+  const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
+  LCM_HIP(hipMemcpyAsync(next->w, cur->w, sz * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
+  LCM_HIP(hipMemcpyAsync(next->z, cur->z, sz * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
+  hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(256), 0, ctx->stream,
+                     out->action + int64_t(period) * n_agents, n_agents, -1);
+  dim3 grid(nb, n_regimes);
+  const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+  if (dtype == LCM_DTYPE_F32) {
+    if (lds_bytes > 64 * 1024)
+      LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(sim_kernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes)));
+    hipLaunchKernelGGL(sim_kernel<float>, grid, dim3(kSimBlock), lds_bytes, ctx->stream, d_sim_descs,
+                       n_regimes, *cur, *next, *out, d_counts, d_order, n_agents, period, k0, k1);
+  } else if (dtype == LCM_DTYPE_F64) {
+    if (lds_bytes > 64 * 1024)
+      LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(sim_kernel<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes)));
+    hipLaunchKernelGGL(sim_kernel<double>, grid, dim3(kSimBlock), lds_bytes, ctx->stream, d_sim_descs,
+                       n_regimes, *cur, *next, *out, d_counts, d_order, n_agents, period, k0, k1);
+  } else {
+    return fail(LCM_E_ARG, "lcm_sim_period: unknown dtype");
+  }
+  LCM_HIP(hipGetLastError());
+  return LCM_OK;
+}

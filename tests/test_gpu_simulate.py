@@ -1,0 +1,114 @@
+"""GPU forward simulation against the oracle (same value functions, same
+counter-based draws): integer draws and chosen actions identical, values and
+continuous states within the dtype's tolerance."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import gridsearch as og
+from oracle import simulate as osim
+from pylcm_amd import _native as nat
+from pylcm_amd import examples as ex
+from tests import models as tm
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_philox_matches_oracle(native):
+    ctx = nat.Context()
+    n = 4096
+    buf = nat.DeviceBuffer(n * 16)
+    for c1, c2, c3, k0, k1 in [(0, 0, 0, 0, 0), (7, 1, 2, 12345, 678), (0xFFFFFFFF, 3, 0, 0xDEADBEEF, 0x1)]:
+        nat.call("lcm_philox", ctx.handle, C.c_void_p(buf.ptr), n, c1, c2, c3, k0, k1)
+        got = np.zeros(4 * n, dtype=np.uint32)
+        nat.d2h(ctx, buf.ptr, got)
+        ctr = np.stack([np.arange(n), np.full(n, c1), np.full(n, c2), np.full(n, c3)], axis=1)
+        exp = osim.philox4x32_10(ctr, (k0, k1)).astype(np.uint32).reshape(-1)
+        assert np.array_equal(got, exp)
+    # Random123 known answer at counter 0, key 0
+    nat.call("lcm_philox", ctx.handle, C.c_void_p(buf.ptr), 1, 0, 0, 0, 0, 0)
+    got = np.zeros(4, dtype=np.uint32)
+    nat.d2h(ctx, buf.ptr, got)
+    assert [int(x) for x in got] == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+
+
+def _compare(model, params, ic, dtype, seed, rtol):
+    V, _ = og.solve(model, params, grid_dtype=dtype)
+    res = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V,
+                         seed=seed, dtype=dtype)
+    got = res.raw
+    exp = osim.simulate(model, params, V, ic, seed, grid_dtype=dtype)
+    sim = exp["action"] >= 0
+    assert np.array_equal(got["action"] >= 0, sim)
+    assert np.array_equal(got["regime"][sim], exp["regime"][sim])
+    assert np.array_equal(got["action"][sim], exp["action"][sim])
+    for k in ("value", "w", "z"):
+        a, b = got[k][sim].astype(np.float64), exp[k][sim]
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin), k
+        err = np.max(np.abs(a[fin] - b[fin]) / np.maximum(np.abs(b[fin]), 1e-12)) if fin.any() else 0
+        print(f"{k}: max rel err {err:.2e}")
+        assert err <= rtol, (k, err)
+    return res
+
+
+def test_simulate_deterministic_f64(native):
+    m, p = ex.deterministic_workload()
+    rng = np.random.default_rng(0)
+    n = 300
+    ic = {"wealth": rng.uniform(1, 400, n), "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
+    res = _compare(m, p, ic, np.float64, seed=11, rtol=1e-10)
+    df = res.to_dataframe()
+    assert list(df.columns[:4]) == ["subject_id", "period", "regime_name", "value"]
+    assert set(df["regime_name"].unique()) <= {"working_life", "retirement", "dead"}
+
+
+def test_simulate_stochastic_f64(native):
+    T = 7
+    m = ex.retirement_model(T, n_wealth=40, n_consumption=50, income_nodes=5)
+    p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.2, "mu": 0.0})
+    rng = np.random.default_rng(1)
+    n = 257
+    ic = {"wealth": rng.uniform(1, 200, n), "income": rng.normal(0, 0.3, n),
+          "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    _compare(m, p, ic, np.float64, seed=5, rtol=1e-9)
+
+
+@pytest.mark.parametrize("dist", ["rouwenhorst", "normal"])
+def test_simulate_processes_model_f64(native, dist):
+    """Markov health (discrete draws) and an income process."""
+    m = tm.processes_model(5, dist)
+    p = tm.processes_params(dist)
+    rng = np.random.default_rng(2)
+    n = 200
+    ic = {"wealth": rng.uniform(1, 5, n), "income": rng.normal(0, 0.5, n),
+          "health": rng.integers(0, 2, n), "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
+    _compare(m, p, ic, np.float64, seed=9, rtol=1e-9)
+
+
+def test_simulate_large_panel_f32(native):
+    """100k agents of the stochastic model: runs, first-period value equals the
+    solve's V where the agent starts on the grid, income moments follow AR(1)."""
+    T = 10
+    m = ex.retirement_model(T, n_wealth=100, n_consumption=100, income_nodes=7)
+    inc = {"rho": 0.95, "sigma": 0.1, "mu": 0.0}
+    p = ex.retirement_params(T, income=inc)
+    n = 100_000
+    rng = np.random.default_rng(3)
+    nodes, P = og.process_nodes_matrix(m.regimes["working_life"].states["income"], inc)
+    pi = np.array([1, 6, 15, 20, 15, 6, 1]) / 64
+    z0 = nodes[rng.choice(7, size=n, p=pi)].astype(np.float32)
+    w0 = og.linspace(1, 400, 100, np.float32)[rng.integers(0, 100, n)]
+    ic = {"wealth": w0, "income": z0, "regime_id": np.zeros(n, dtype=int)}
+    V, _ = m.solve(params=p, log_level="off", dtype=np.float32, return_policy=True)
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=2026,
+                     dtype=np.float32)
+    raw = res.raw
+    assert np.all(raw["action"][0] >= 0)
+    wi = np.rint((w0 - 1) / (399 / 99)).astype(int)
+    zi = np.searchsorted(nodes.astype(np.float32), z0)
+    assert np.allclose(raw["value"][0], V[0]["working_life"][zi, wi], rtol=1e-6)
+    z1 = raw["z"][1].astype(np.float64)
+    assert abs(z1.mean() - 0.95 * z0.mean()) < 0.005
