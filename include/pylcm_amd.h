@@ -152,6 +152,14 @@ typedef struct LcmSimDesc {
   const void* g_pts;     /* next continuous grid */
   const void* preg;      /* [n_sdd][n_d][n_regimes] regime probabilities */
   const void* Wd[LCM_SIM_MAX_REGIMES]; /* per target regime [n_sdd][n_d][n_sdd_r] */
+  /* one constraint per side may read the agent's features:
+   *   lo_eff = max(lo, LoB + sum_k LoF_k*phi_k),  hi_eff = min(hi, HiB + sum_k HiF_k*phi_k)
+   * (null: absent; a strict side takes the next representable number) */
+  const void* LoB;
+  const void* HiB;
+  const void* LoF[LCM_SIM_MAX_FEAT];
+  const void* HiF[LCM_SIM_MAX_FEAT];
+  int32_t lo_strict, hi_strict;
   int32_t n_sdd_next[LCM_SIM_MAX_REGIMES];
   int32_t feat_op[LCM_SIM_MAX_FEAT];
   int32_t n_feat, n_sdd, n_d, n_c;

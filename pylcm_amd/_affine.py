@@ -253,20 +253,22 @@ def _as_aff(x):
 class Constraint:
     """Conjunction of affine inequalities `a_k * w + b_k >= 0` (or `> 0`)."""
 
-    def __init__(self, terms):
+    def __init__(self, terms, feats=None):
         self.terms = list(terms)  # [(a, b, strict)] ; a concrete bool mask is (0, 0|-inf, False)
+        # per term: {feature id: coefficient} added to b (forward simulation only)
+        self.feats = list(feats) if feats is not None else [{} for _ in self.terms]
 
     @staticmethod
     def make(expr, strict):
         expr = _as_aff(expr)
         if expr.is_const:
             return (expr.b > 0) if strict else (expr.b >= 0)
-        if expr.has_feats:
-            raise NonAffine("constraint depends on a per-agent process value")
-        return Constraint([(expr.a, expr.b, strict)])
+        if expr.has_feats and not expr.has_w:
+            raise NonAffine("constraint on a per-agent process value alone")
+        return Constraint([(expr.a, expr.b, strict)], [dict(expr.feats)])
 
     def __and__(self, o):
-        return Constraint(self.terms + _constraint_terms(o))
+        return Constraint(self.terms + _constraint_terms(o), self.feats + _constraint_feats(o))
 
     __rand__ = __and__
 
@@ -275,10 +277,11 @@ class Constraint:
 
     def __array_ufunc__(self, ufunc, method, *inputs, **kwargs):
         if ufunc in (np.logical_and, np.bitwise_and) and method == "__call__":
-            terms = []
+            terms, feats = [], []
             for i in inputs:
                 terms += _constraint_terms(i)
-            return Constraint(terms)
+                feats += _constraint_feats(i)
+            return Constraint(terms, feats)
         raise NonAffine(f"{ufunc.__name__} of a state-dependent constraint")
 
     def __array_function__(self, func, types, args, kwargs):
@@ -290,3 +293,7 @@ def _constraint_terms(x):
         return x.terms
     mask = np.asarray(x, dtype=bool)
     return [(np.zeros(()), np.where(mask, 0.0, -np.inf), False)]
+
+
+def _constraint_feats(x):
+    return x.feats if isinstance(x, Constraint) else [{}]

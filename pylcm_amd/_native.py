@@ -84,6 +84,9 @@ class LcmSimDesc(C.Structure):
         ("fcoef", vp * SIM_MAX_FEAT), ("fconst", vp * SIM_MAX_FEAT),
         ("EV", vp), ("CS", vp), ("sd_base", vp), ("nodes", vp), ("g_pts", vp), ("preg", vp),
         ("Wd", vp * SIM_MAX_REGIMES),
+        ("LoB", vp), ("HiB", vp),
+        ("LoF", vp * SIM_MAX_FEAT), ("HiF", vp * SIM_MAX_FEAT),
+        ("lo_strict", C.c_int32), ("hi_strict", C.c_int32),
         ("n_sdd_next", C.c_int32 * SIM_MAX_REGIMES),
         ("feat_op", C.c_int32 * SIM_MAX_FEAT),
         ("n_feat", C.c_int32), ("n_sdd", C.c_int32), ("n_d", C.c_int32), ("n_c", C.c_int32),

@@ -65,6 +65,12 @@ class SimTables:
     preg: np.ndarray | None = None       # [n_sdd, n_d, n_regimes]
     Wd: dict = field(default_factory=dict)  # target regime -> [n_sdd, n_d, n_sdd_r]
     interp: bool = False
+    LoB: np.ndarray | None = None
+    HiB: np.ndarray | None = None
+    LoF: dict = field(default_factory=dict)
+    HiF: dict = field(default_factory=dict)
+    lo_strict: bool = False
+    hi_strict: bool = False
 
 
 class SimLayout:
@@ -141,6 +147,7 @@ class SimLowering(PeriodLowering):
         U0 = self._to3(sl, u.b if isinstance(u, Affine) else u, shape)
         ufeats = u.feats if isinstance(u, Affine) else {}
         rows = []
+        feat_rows = []  # (a, b, strict, {fid: coef}) of rows reading per-agent features
         for cname, cf in reg.constraints.items():
             if cf is None:
                 continue
@@ -149,7 +156,11 @@ class SimLowering(PeriodLowering):
             except NonAffine as e:
                 raise UnsupportedModelError(f"constraint {cname!r} is not simulatable here ({e})") from e
             if isinstance(out, Constraint):
-                for a, b, strict in out.terms:
+                for (a, b, strict), fts in zip(out.terms, out.feats):
+                    if fts and any(np.any(v != 0) for v in fts.values()):
+                        feat_rows.append((self._to3(sl, a, shape), self._to3(sl, b, shape), strict,
+                                          {k: self._to3(sl, v, shape) for k, v in fts.items()}))
+                        continue
                     rows.append((self._to3(sl, a, shape)[:, None], self._to3(sl, b, shape)[:, None], strict))
             else:
                 m = self._to3(sl, np.asarray(out, dtype=bool), shape)
@@ -163,6 +174,19 @@ class SimLowering(PeriodLowering):
         st = SimTables(regime=lay.name, period=period, terminal=lay.terminal, n_sdd=sl.n_sdd,
                        n_d=lay.n_d, n_c=lay.n_c, U0=U0, Uf=[], A=None, B0=None, Bf=[],
                        lo=np.ascontiguousarray(lo), hi=np.ascontiguousarray(hi), feats=[])
+        # a row a*w + b + sum_k f_k phi_k >= 0 with a = +1 bounds w from below
+        # (w >= -b - sum f phi), with a = -1 from above (w <= b + sum f phi)
+        for a, b, strict, fts in feat_rows:
+            if np.all(a == 1):
+                if st.LoB is not None:
+                    raise UnsupportedModelError("two lower bounds read per-agent values")
+                st.LoB, st.LoF, st.lo_strict = -b, {k: -v for k, v in fts.items()}, strict
+            elif np.all(a == -1):
+                if st.HiB is not None:
+                    raise UnsupportedModelError("two upper bounds read per-agent values")
+                st.HiB, st.HiF, st.hi_strict = b, dict(fts), strict
+            else:
+                raise UnsupportedModelError("a constraint reading per-agent values needs coefficient +-1 on w")
         bfeats = {}
         if not lay.terminal and lay.w_name is not None:
             law = reg.state_transitions.get(lay.w_name)
@@ -445,7 +469,7 @@ class SimProgram:
                 keep_tables.append(st)
                 sim_desc[(t, r)] = self._desc(st, sl, t, r, ar, zero, ev_meta)
                 g_n = ev_meta.get((t, r), (0, 0, 0))[2]
-                ntab = st.n_sdd * (5 + 2 * len(st.feats))
+                ntab = st.n_sdd * (7 + 4 * len(st.feats))
                 L = nat.load_library()
                 code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
                 lds_max = max(lds_max, int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab)))
@@ -472,13 +496,14 @@ class SimProgram:
         for (t, r), d in sim_desc.items():
             rec = SD[t * R + model.regime_code(r)]
             for k, v in d.items():
-                if k in ("Uf", "Bf", "fcoef", "fconst", "Wd"):
+                if k in ("Uf", "Bf", "fcoef", "fconst", "Wd", "LoF", "HiF"):
                     for j, p in enumerate(v):
                         rec[k][j] = (b2 + p) if p is not None else 0
                 elif k in ("n_sdd_next", "feat_op"):
                     for j, p in enumerate(v):
                         rec[k][j] = p
-                elif k in ("U0", "A", "B0", "lo", "hi", "EV", "CS", "sd_base", "nodes", "g_pts", "preg"):
+                elif k in ("U0", "A", "B0", "lo", "hi", "EV", "CS", "sd_base", "nodes", "g_pts", "preg",
+                           "LoB", "HiB"):
                     rec[k] = (b2 + v) if v is not None else 0
                 else:
                     rec[k] = v
@@ -539,6 +564,12 @@ class SimProgram:
         d["B0"] = ar.add(st.B0, dtype=dt) if st.B0 is not None else zero
         d["Uf"] = [ar.add(u, dtype=dt) if u is not None else None for u in st.Uf]
         d["Bf"] = [ar.add(b, dtype=dt) if b is not None else None for b in st.Bf]
+        nf = len(st.feats)
+        d["LoB"] = ar.add(st.LoB, dtype=dt) if st.LoB is not None else None
+        d["HiB"] = ar.add(st.HiB, dtype=dt) if st.HiB is not None else None
+        d["LoF"] = [ar.add(st.LoF[k], dtype=dt) if k in st.LoF else None for k in range(nf)]
+        d["HiF"] = [ar.add(st.HiF[k], dtype=dt) if k in st.HiF else None for k in range(nf)]
+        d["lo_strict"], d["hi_strict"] = int(st.lo_strict), int(st.hi_strict)
         d["fcoef"] = [ar.add(c, dtype=dt) for _, c, _ in st.feats]
         d["fconst"] = [ar.add(c, dtype=dt) for _, _, c in st.feats]
         d["feat_op"] = [op for op, _, _ in st.feats]

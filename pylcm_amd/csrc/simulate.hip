@@ -246,7 +246,9 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   // LDS: EV rows of the current d [n_rows][g_n], grid points, action chunk tables
   T* ev = reinterpret_cast<T*>(smem);
   T* gp = ev + (n_rows * g_n > 0 ? n_rows * g_n : 1);
-  const int ntab = 5 + 2 * nf;  // U0 A B0 lo hi Uf.. Bf..
+  // staged tables per action: U0 A B0 lo hi LoB HiB Uf.. Bf.. LoF.. HiF..
+  const int ntab = 7 + 4 * nf;
+  const bool lo_feat = D->LoB != nullptr, hi_feat = D->HiB != nullptr;
   T* tab = gp + (g_n > 0 ? g_n : 1);
   if (interp && D->g_kind == LCM_GRID_IRREG) {
     for (int k = threadIdx.x; k < g_n; k += kSimBlock) gp[k] = static_cast<const T*>(D->g_pts)[k];
@@ -284,24 +286,41 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
         base[2 * kSimChunk] = interp ? static_cast<const T*>(D->B0)[e] : T(0);
         base[3 * kSimChunk] = static_cast<const T*>(D->lo)[e];
         base[4 * kSimChunk] = static_cast<const T*>(D->hi)[e];
+        base[5 * kSimChunk] = lo_feat ? static_cast<const T*>(D->LoB)[e] : T(-INFINITY);
+        base[6 * kSimChunk] = hi_feat ? static_cast<const T*>(D->HiB)[e] : T(INFINITY);
         for (int k = 0; k < nf; ++k) {
-          base[(5 + k) * kSimChunk] = D->Uf[k] ? static_cast<const T*>(D->Uf[k])[e] : T(0);
-          base[(5 + nf + k) * kSimChunk] = (interp && D->Bf[k]) ? static_cast<const T*>(D->Bf[k])[e] : T(0);
+          base[(7 + k) * kSimChunk] = D->Uf[k] ? static_cast<const T*>(D->Uf[k])[e] : T(0);
+          base[(7 + nf + k) * kSimChunk] = (interp && D->Bf[k]) ? static_cast<const T*>(D->Bf[k])[e] : T(0);
+          base[(7 + 2 * nf + k) * kSimChunk] = (lo_feat && D->LoF[k]) ? static_cast<const T*>(D->LoF[k])[e] : T(0);
+          base[(7 + 3 * nf + k) * kSimChunk] = (hi_feat && D->HiF[k]) ? static_cast<const T*>(D->HiF[k])[e] : T(0);
         }
       }
       __syncthreads();
       const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
       for (int i = 0; i < cn; ++i) {
-        const bool feas = valid && (w >= mine[3 * kSimChunk + i]) && (w <= mine[4 * kSimChunk + i]);
+        T lo = mine[3 * kSimChunk + i], hi = mine[4 * kSimChunk + i];
+        if (lo_feat) {
+          T l = mine[5 * kSimChunk + i];
+          for (int k = 0; k < nf; ++k) l += mine[(7 + 2 * nf + k) * kSimChunk + i] * phi[k];
+          if (D->lo_strict) l = nextafter(l, T(INFINITY));
+          lo = fmax(lo, l);
+        }
+        if (hi_feat) {
+          T h = mine[6 * kSimChunk + i];
+          for (int k = 0; k < nf; ++k) h += mine[(7 + 3 * nf + k) * kSimChunk + i] * phi[k];
+          if (D->hi_strict) h = nextafter(h, T(-INFINITY));
+          hi = fmin(hi, h);
+        }
+        const bool feas = valid && (w >= lo) && (w <= hi);
         if (__ballot(feas) == 0) continue;
         T u = mine[i];
-        for (int k = 0; k < nf; ++k) u += mine[(5 + k) * kSimChunk + i] * phi[k];
+        for (int k = 0; k < nf; ++k) u += mine[(7 + k) * kSimChunk + i] * phi[k];
         T q;
         if (D->terminal) {
           q = u;
         } else if (interp) {
           T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
-          for (int k = 0; k < nf; ++k) x += mine[(5 + nf + k) * kSimChunk + i] * phi[k];
+          for (int k = 0; k < nf; ++k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
           const T coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
           const T v_lo = interp_at(ev + int64_t(row_lo) * g_n, g_n, coord);
           T v = zero_safe(T(1) - tz, v_lo);
