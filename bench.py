@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--draws N]
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -37,6 +38,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "gs_period_kernel_f32_1024draws_T40_traffic.md"
+
+
+def committed_traffic(key: str):
+    """HBM bytes per launch of the dominant kernel, from the committed summary
+    of two counter runs (FETCH_SIZE, WRITE_SIZE) of this same workload, each
+    scaled by what the counter reads for a copy kernel of known size. None when
+    the summary is absent or was taken on another workload."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    fields = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("- ") and ": " in line:
+            k, v = line[2:].split(": ", 1)
+            fields[k.strip()] = v.strip().strip("`")
+    if fields.get("workload_key") != key:
+        return None
+    try:
+        return int(fields["fetch_bytes_per_launch"]) + int(fields["write_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +72,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
+    ap.add_argument("--copy-calib-mib", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -72,6 +97,15 @@ def main():
     prog = GsProgram.of(problems, dt, lowered=lowered, tile=args.tile)
     log(f"packed {prog.total_bytes/2**20:.1f} MiB arena, {prog.n_desc} descriptors, tile {prog.tile},"
         f" in {time.perf_counter()-t0:.1f}s")
+
+    # ---- copy kernel of known size (calibrates the HBM traffic counters) -----
+    calib = args.copy_calib_mib * 2**20
+    if calib:
+        src, dst = nat.DeviceBuffer(calib), nat.DeviceBuffer(calib)
+        for _ in range(2):
+            nat.call("lcm_copy_launch", prog.ctx.handle, dst.ptr, src.ptr, C.c_size_t(calib))
+        prog.ctx.sync()
+        del src, dst
 
     # ---- batched: warmup + timed graph replays --------------------------------
     prog.capture_graph()
@@ -143,16 +177,16 @@ def main():
     parity = {}
     if not args.no_cpu:
         from oracle import cpu_solve
-        from tests.parity import compare_solution
-        from oracle import gridsearch as og
 
         threads = len(os.sched_getaffinity(0))
         # a bounded sample with the batch's size mix: the first draw of each size
         order = []
         for size in sorted({s for s, _ in draws}):
             order += [i for i in range(args.draws) if draws[i][0] == size][: max(1, args.cpu_sample // 3)]
+        from tests.parity import rel_err
+
         cpu_t = 0.0
-        worst, mism, ties = 0.0, 0, 0
+        worst, mism, ties, tie_gap = 0.0, 0, 0, 0.0
         for i in order:
             s, p = draws[i]
             t0 = time.perf_counter()
@@ -161,20 +195,30 @@ def main():
             rb = prog.results(i)
             Vg = {t: {r: v for r, (v, _) in rr.items()} for t, rr in rb.items()}
             Pg = {t: {r: q for r, (_, q) in rr.items()} for t, rr in rb.items()}
-            from tests.parity import rel_err
             for t in Vc:
                 for r in Vc[t]:
                     worst = max(worst, rel_err(Vg[t][r], Vc[t][r]))
-                    mism += int(np.sum(Pg[t][r] != Pc[t][r]))
+            # argmax check per period from the same inputs: the FP64 step reads
+            # the GPU's V_{t+1}; a differing index is a flagged near-tie when
+            # FP64 Q at the two actions is within 8 float32 ulp
+            Vg64 = {t: {r: np.asarray(v, dtype=np.float64) for r, v in rr.items()} for t, rr in Vg.items()}
+            _, _, rep = cpu_solve.solve_c(models[s], p, grid_dtype=dt, threads=threads,
+                                          next_V=Vg64, cmp_policy=Pg)
+            mism += rep["mismatches"]
+            ties += rep["near_ties"]
+            tie_gap = max(tie_gap, rep["max_tie_gap_ulps"])
         cpu = {"value": len(order) / cpu_t, "unit": "model-solves/s", "cores": threads,
                "kind": "port",
                "sample": f"{len(order)} draws of the batch (grid sizes "
                          f"{sorted(draws[i][0] for i in order)}), C restatement oracle/gs_ref.c "
                          f"with OpenMP, FP64, {cpu_t:.1f}s"}
-        parity = {"max_rel_V_err_vs_fp64": worst, "policy_mismatches_incl_near_ties": mism,
+        parity = {"max_rel_V_err_vs_fp64": worst, "policy_mismatches": mism,
+                  "flagged_near_ties": ties, "max_near_tie_gap_f32_ulps": round(tie_gap, 2),
                   "checked_draws": len(order)}
         log(f"cpu baseline: {cpu['value']:.3f} solves/s on {threads} threads; parity {parity}")
 
+    key = f"draws={args.draws} periods={T} tile={prog.tile} dtype=f32"
+    traffic = committed_traffic(key)
     out = {
         "metric": "model-solves/sec (batched vs plain GPU), p50/p99 solve latency, V rel err <= 1e-5",
         "value": round(value, 3),
@@ -209,8 +253,9 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
-            "kernel": "gs_period_kernel",
+            "traffic": traffic,
+            "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None,
+            "kernel": f"gs_period_kernel<float, {prog.tile}, {max(prog.wdep_by_step)}>",
             "avg_launch_ms": round(avg_ms, 4),
             "launches": len(per),
             "gflop_per_launch": round(flops_per_launch / 1e9, 3),

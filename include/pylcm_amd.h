@@ -237,6 +237,12 @@ int lcm_event_destroy(void* ev);
 int lcm_event_record(lcm_ctx* ctx, void* ev);
 int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms);
 
+/* Plain device-to-device copy kernel of `bytes` (a multiple of 16) on ctx's
+ * stream. No counterpart in the reference: a kernel of known traffic whose
+ * FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic read for the
+ * solve kernels. */
+int lcm_copy_launch(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 /* Dynamic LDS bytes lcm_gs_launch needs when the launch's largest next-state
  * grid has max_g points and its descriptors have at most max_constr rows. */
 size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr);

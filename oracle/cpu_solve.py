@@ -33,8 +33,20 @@ def _dp(a):
     return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
 
 
-def solve_c(model, params, *, grid_dtype=np.float64, threads=0):
+def solve_c(model, params, *, grid_dtype=np.float64, threads=0, next_V=None, cmp_policy=None,
+            tie_ulps=8):
+    """FP64 backward induction with the C Bellman step. Returns (V, POL).
+
+    With `next_V` (period -> regime -> V) each period reads that V for period
+    t+1 instead of its own, so a solve computed elsewhere is re-checked one
+    period at a time from the same inputs. With `cmp_policy` the return is
+    (V, POL, report): `report` counts the policy entries that differ from
+    `cmp_policy` as near-ties (this step's FP64 Q at the two actions within
+    `tie_ulps` units in the last place of `grid_dtype`) or genuine mismatches.
+    """
     lib = _load()
+    report = {"mismatches": 0, "near_ties": 0, "max_tie_gap_ulps": 0.0}
+    eps = float(np.finfo(np.dtype(grid_dtype)).eps)
     gdt = np.dtype(grid_dtype)
     fixed = model.fixed_params
     names = list(model.regime_id_class.__categories__)
@@ -49,7 +61,8 @@ def solve_c(model, params, *, grid_dtype=np.float64, threads=0):
             reg = R[n]
             if not og._active(reg.regime, age):
                 continue
-            tab = og.tabulate(model, R, reg, t, params, fixed, gdt, V.get(t + 1))
+            Vnext = next_V.get(t + 1) if next_V is not None else V.get(t + 1)
+            tab = og.tabulate(model, R, reg, t, params, fixed, gdt, Vnext)
             shape = tab["shape"]
             nsd = int(np.prod([len(v) for v in reg.sd_vals])) if reg.sd_vals else 1
             nw = len(reg.w_pts) if reg.w else 1
@@ -87,11 +100,31 @@ def solve_c(model, params, *, grid_dtype=np.float64, threads=0):
             taste = tab["taste"] or 0.0
             Vout = np.zeros(nsd * nw)
             pol = np.zeros(nsd * nw * (nd if taste > 0 else 1), dtype=np.int32)
+            pcmp = qcmp = None
+            if cmp_policy is not None:
+                pcmp = np.ascontiguousarray(np.asarray(cmp_policy[t][n]).reshape(-1), dtype=np.int32)
+                assert pcmp.size == pol.size
+                qcmp = np.zeros(2 * pol.size)
             lib.gs_step_ref(nsd, nw, nd, nc, _dp(U), F.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(wn),
                             int(tab["terminal"]), len(tg), Vt, Kt, Wt, pt, len(sc), _dp(vs), qs,
                             gkind, G, _dp(gpts), C.c_double(tab.get("beta", 0.0) or 0.0),
                             C.c_double(taste), _dp(Vout) if False else Vout.ctypes.data_as(C.POINTER(C.c_double)),
-                            pol.ctypes.data_as(C.POINTER(C.c_int32)), int(threads))
+                            pol.ctypes.data_as(C.POINTER(C.c_int32)),
+                            None if pcmp is None else pcmp.ctypes.data_as(C.POINTER(C.c_int32)),
+                            None if qcmp is None else qcmp.ctypes.data_as(C.POINTER(C.c_double)),
+                            int(threads))
+            if pcmp is not None:
+                bad = np.nonzero(pcmp != pol)[0]
+                q_at, q_best = qcmp[0::2][bad], qcmp[1::2][bad]
+                with np.errstate(invalid="ignore"):
+                    gap = np.abs(q_best - q_at) / (eps * np.maximum(np.maximum(np.abs(q_best), np.abs(q_at)), 1.0))
+                tie = np.isfinite(gap) & (gap <= tie_ulps)
+                report["mismatches"] += int(np.sum(~tie))
+                report["near_ties"] += int(np.sum(tie))
+                if np.any(tie):
+                    report["max_tie_gap_ulps"] = max(report["max_tie_gap_ulps"], float(np.max(gap[tie])))
             V[t][n] = Vout.reshape(reg.V_shape)
             POL[t][n] = pol.reshape(reg.V_shape + ((nd,) if taste > 0 else ()))
+    if cmp_policy is not None:
+        return V, POL, report
     return V, POL

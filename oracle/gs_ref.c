@@ -70,13 +70,17 @@ static double interp(const double* row, int G, double coord) {
  * n_s stateless targets: v_s value, q_s [n_sd*n_d] probabilities
  * terminal: Q = U. taste > 0: logsum over d of the per-d max.
  * Vout [S]; pol [S] or [S*n_d] with taste shocks.
+ * pcmp / qcmp (optional, NULL to skip): for a policy to compare against (same
+ * layout as pol), qcmp[2 * slot] receives this step's Q at that policy's action
+ * and qcmp[2 * slot + 1] the best Q of the slot, so a caller can tell a near-tie
+ * from a genuine argmax mismatch.
  */
 int gs_step_ref(int n_sd, int n_w, int n_d, int n_c, const double* U, const uint8_t* F,
                 const double* wn, int terminal, int n_t, const double* const* Vt,
                 const int* Kt, const double* const* Wt, const double* const* pt, int n_s,
                 const double* vs, const double* const* qs, int gkind, int G,
                 const double* gpts, double beta, double taste, double* Vout, int32_t* pol,
-                int threads) {
+                const int32_t* pcmp, double* qcmp, int threads) {
   const long S = (long)n_sd * n_w;
   const long A = (long)n_d * n_c;
 #ifdef _OPENMP
@@ -108,6 +112,9 @@ int gs_step_ref(int n_sd, int n_w, int n_d, int n_c, const double* U, const uint
       }
       double dbest = -INFINITY;
       int dseen = 0, dnan = 0, didx = 0;
+      const long cmp_slot = taste > 0 ? s * n_d + d : s;
+      const int cmp_idx = pcmp ? pcmp[cmp_slot] : -1;
+      if (qcmp && (taste > 0 || d == 0)) qcmp[2 * cmp_slot] = -INFINITY;
       for (int c = 0; c < n_c; ++c) {
         const long i = s * A + (long)d * n_c + c;
         if (!F[i]) continue;
@@ -134,9 +141,11 @@ int gs_step_ref(int n_sd, int n_w, int n_d, int n_c, const double* U, const uint
           ce = ce / (ok ? mass : NAN);
           q = U[i] + beta * ce;
         }
+        if (qcmp && d * n_c + c == cmp_idx) qcmp[2 * cmp_slot] = q;
         if (q != q) { dnan = 1; continue; }
         if (!dseen || q > dbest) { dbest = q; didx = d * n_c + c; dseen = 1; }
       }
+      if (qcmp && taste > 0) qcmp[2 * cmp_slot + 1] = dnan ? NAN : (dseen ? dbest : -INFINITY);
       if (taste > 0) {
         qc_best[d] = dnan ? NAN : (dseen ? dbest : -INFINITY);
         qc_idx[d] = (dseen && !dnan) ? didx : 0;
@@ -155,6 +164,7 @@ int gs_step_ref(int n_sd, int n_w, int n_d, int n_c, const double* U, const uint
       for (int d = 0; d < n_d; ++d) pol[s * n_d + d] = qc_idx[d];
     } else {
       Vout[s] = nan ? NAN : (seen ? best : -INFINITY);
+      if (qcmp) qcmp[2 * s + 1] = Vout[s];
       pol[s] = (seen && !nan) ? bidx : 0;
     }
   }

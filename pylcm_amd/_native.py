@@ -132,6 +132,7 @@ _SIGS = {
     "lcm_event_destroy": ([vp], C.c_int),
     "lcm_event_record": ([vp, vp], C.c_int),
     "lcm_event_elapsed": ([vp, vp, C.POINTER(C.c_float)], C.c_int),
+    "lcm_copy_launch": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_abi_sizes": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
     "lcm_gs_lds_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),
     "lcm_abi_sizes_sim": ([C.POINTER(C.c_size_t)] * 4, C.c_int),

@@ -211,6 +211,29 @@ int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms) {
 
 }  // extern "C"
 
+// 16-byte grid-stride copy; traffic is exactly `n16 * 16` bytes read and
+// written, which makes it the calibration point for the HBM counters.
+__global__ void __launch_bounds__(256) lcm_copy_kernel(uint4* __restrict__ dst,
+                                                       const uint4* __restrict__ src,
+                                                       size_t n16) {
+  size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+extern "C" int lcm_copy_launch(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_copy_launch: null argument");
+  if (bytes % 16) return fail(LCM_E_ARG, "lcm_copy_launch: bytes must be a multiple of 16");
+  if (bytes == 0) return LCM_OK;
+  const size_t n16 = bytes / 16;
+  size_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(lcm_copy_kernel, dim3(unsigned(blocks)), dim3(256), 0, ctx->stream,
+                     static_cast<uint4*>(dst), static_cast<const uint4*>(src), n16);
+  LCM_HIP(hipGetLastError());
+  return LCM_OK;
+}
+
 // Sizes of the structs shared with the host binding, for layout checks.
 extern "C" int lcm_abi_sizes(size_t* gs_desc, size_t* gs_task) {
   if (gs_desc) *gs_desc = sizeof(LcmGsDesc);
