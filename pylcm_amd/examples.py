@@ -11,8 +11,11 @@
   - stochastic, float32, T=40, wealth 500 x consumption 500, 7 income nodes;
   - batched draws of the stochastic model with grid sizes in {200, 500, 1000}.
 
-* `dcegm_retirement_model` — the same economics for the DC-EGM solver with EV1
-  taste shocks and a log-normal income shock on Gauss-Hermite nodes.
+* DC-EGM models: `iskhakov_dcegm_model` (the deterministic retirement model
+  solved by DC-EGM), `dcegm_twin_model` (twin of the independent `dcegm`
+  package's taste-shock model the reference cross-checks against) and
+  `dcegm_retirement_model` (CRRA, taste shocks, log-normal income level on
+  Gauss-Hermite nodes; the DC-EGM benchmark workload).
 """
 
 from __future__ import annotations
@@ -29,6 +32,7 @@ from pylcm_amd import (
     IrregSpacedGrid,
     LinSpacedGrid,
     LiquidMargin,
+    LogSpacedGrid,
     LogNormalIIDProcess,
     Model,
     Regime,
@@ -228,7 +232,7 @@ def draw_batch_params(n_draws, n_periods=40, seed=0):
 
 
 # ---------------------------------------------------------------------------
-# DC-EGM variant
+# DC-EGM models
 # ---------------------------------------------------------------------------
 
 
@@ -236,8 +240,210 @@ def savings(wealth, consumption):
     return wealth - consumption
 
 
-def next_wealth_from_savings(savings, labor_income, income, interest_rate):
+def next_wealth_from_savings(savings, labor_income, interest_rate):
+    """`(1 + r) * (wealth - consumption) + labor_income` written in the savings."""
+    return (1 + interest_rate) * savings + labor_income
+
+
+def inverse_marginal_utility_log(marginal_continuation):
+    """Inverse of u'(c) = 1/c (work disutility is additive)."""
+    return 1.0 / marginal_continuation
+
+
+def inverse_marginal_utility_crra(marginal_continuation, crra):
+    """Inverse of u'(c) = c**(-crra)."""
+    return marginal_continuation ** (-1.0 / crra)
+
+
+def iskhakov_dcegm_model(n_periods: int, *, n_savings: int = 200, n_constrained_points: int = 64,
+                         n_wealth: int = 100) -> Model:
+    """DC-EGM twin of `retirement_model(log_utility=True)`: the deterministic
+    Iskhakov et al. (2017) retirement model with a cubic savings grid on
+    [0, 400] (`tests/test_models/deterministic/dcegm_variants.py:39-140` of the
+    reference)."""
+    last_age = n_periods - 1
+    W = LinSpacedGrid(start=1.0, stop=400.0, n_points=n_wealth)
+    Cg = LinSpacedGrid(start=1.0, stop=400.0, n_points=500)
+    sav = IrregSpacedGrid(points=tuple(400.0 * (i / (n_savings - 1)) ** 3 for i in range(n_savings)))
+    solver = DCEGM(savings_grid=sav, n_constrained_points=n_constrained_points)
+    margin = LiquidMargin(state="wealth", action="consumption", resources="wealth",
+                          post_decision_state="savings")
+    working = ConsumptionSavingsRegime(
+        actions={"labor_supply": DiscreteGrid(LaborSupply), "consumption": Cg},
+        states={"wealth": W}, state_transitions={"wealth": next_wealth_from_savings},
+        transition=next_regime_from_working,
+        functions={"utility": log_utility_working, "labor_income": labor_income,
+                   "is_working": is_working, "savings": savings,
+                   "inverse_marginal_utility": inverse_marginal_utility_log},
+        solver=solver, liquid=margin, active=lambda age, la=last_age: age < la)
+    retired = ConsumptionSavingsRegime(
+        actions={"consumption": Cg}, states={"wealth": W},
+        state_transitions={"wealth": next_wealth_from_savings},
+        transition=next_regime_from_retirement,
+        functions={"utility": log_utility_retirement, "savings": savings,
+                   "inverse_marginal_utility": inverse_marginal_utility_log},
+        solver=solver, liquid=margin, active=lambda age, la=last_age: age < la)
+    return Model(regimes={"working_life": working, "retirement": retired, "dead": _dead()},
+                 ages=AgeGrid(start=0, stop=last_age, step="Y"), regime_id_class=RegimeId)
+
+
+# -- twin of the independent `dcegm` package's IJRS taste-shock model ----------
+# (reference: tests/test_models/dcegm_paper_twin.py; fixture
+#  tests/data/dcegm_reference/ijrs_taste_shocks_reference.csv)
+
+TWIN_MIN_AGE = 20
+TWIN_N_PERIODS = 10
+TWIN_LAST_ALIVE_AGE = TWIN_MIN_AGE + TWIN_N_PERIODS - 2
+TWIN_WEALTH_POINTS = (
+    tuple(0.001 * (0.5 / 0.001) ** (i / 120) for i in range(120))
+    + tuple(0.5 + 0.05 * i for i in range(90))
+    + tuple(5.0 + 0.25 * i for i in range(181))
+)
+
+
+@categorical(ordered=False)
+class TwinRegimeId:
+    working_life: int
+    retirement: int
+    done_from_working: int
+    done_retired: int
+
+
+@categorical(ordered=True)
+class WorkChoice:
+    work: int
+    retire: int
+
+
+def twin_crra(consumption, rho):
+    return (consumption ** (1.0 - rho) - 1.0) / (1.0 - rho)
+
+
+def twin_is_working(work_choice):
+    return work_choice == WorkChoice.work
+
+
+def twin_utility_working(consumption, is_working, rho, delta):
+    return twin_crra(consumption, rho) - np.where(is_working, delta, 0.0)
+
+
+def twin_utility_retired(consumption, rho):
+    return twin_crra(consumption, rho)
+
+
+def twin_utility_done_from_working(wealth, rho, delta, taste_scale):
+    return twin_crra(wealth, rho) + taste_scale * np.log(1.0 + np.exp(-delta / taste_scale))
+
+
+def twin_utility_done_retired(wealth, rho):
+    return twin_crra(wealth, rho)
+
+
+def twin_labor_income(is_working, age, constant, exp_coeff, exp_squared):
+    next_age = age + 1.0
+    return np.where(is_working, np.exp(constant + exp_coeff * next_age + exp_squared * next_age**2), 0.0)
+
+
+def twin_next_wealth(wealth, consumption, labor_income, interest_rate, floor):
+    return np.maximum((1.0 + interest_rate) * (wealth - consumption) + labor_income, floor)
+
+
+def twin_next_wealth_from_savings(savings, labor_income, interest_rate, floor):
+    return np.maximum((1.0 + interest_rate) * savings + labor_income, floor)
+
+
+def twin_inverse_marginal_utility(marginal_continuation, rho):
+    return marginal_continuation ** (-1.0 / rho)
+
+
+def twin_next_regime_from_working(work_choice, age):
+    last = age >= TWIN_LAST_ALIVE_AGE
+    retire = work_choice == WorkChoice.retire
+    return np.where(last, np.where(retire, TwinRegimeId.done_retired, TwinRegimeId.done_from_working),
+                    np.where(retire, TwinRegimeId.retirement, TwinRegimeId.working_life))
+
+
+def twin_next_regime_from_retirement(age):
+    return np.where(age >= TWIN_LAST_ALIVE_AGE, TwinRegimeId.done_retired, TwinRegimeId.retirement)
+
+
+def dcegm_twin_model(*, grid_search: bool = False, savings_points=None) -> Model:
+    """The IJRS consumption-retirement model with EV1 taste shocks on the
+    work/retire choice, absorbing retirement and two terminal regimes holding
+    the closed-form final-period values. DC-EGM on a 500-point uniform savings
+    grid on [0, 50], or grid search over 400 log-spaced consumption points."""
+    W = IrregSpacedGrid(points=TWIN_WEALTH_POINTS)
+    Cg = LogSpacedGrid(start=0.001, stop=50.0, n_points=400)
+    active = lambda age: age <= TWIN_LAST_ALIVE_AGE  # noqa: E731
+    if savings_points is None:
+        sav = LinSpacedGrid(start=0.0, stop=50.0, n_points=500)
+    else:
+        sav = IrregSpacedGrid(points=tuple(float(x) for x in savings_points))
+    solver = DCEGM(savings_grid=sav)
+    margin = LiquidMargin(state="wealth", action="consumption", resources="wealth",
+                          post_decision_state="savings")
+    fw = {"utility": twin_utility_working, "is_working": twin_is_working,
+          "labor_income": twin_labor_income}
+    fr = {"utility": twin_utility_retired}
+    acts_w = {"work_choice": DiscreteGrid(WorkChoice), "consumption": Cg}
+    if grid_search:
+        working = Regime(transition=twin_next_regime_from_working, states={"wealth": W}, actions=acts_w,
+                         taste_shocks=ExtremeValueTasteShocks(), active=active,
+                         state_transitions={"wealth": twin_next_wealth},
+                         constraints={"borrowing_constraint": borrowing_constraint}, functions=fw)
+        retired = Regime(transition=twin_next_regime_from_retirement, states={"wealth": W},
+                         actions={"consumption": Cg}, active=active,
+                         state_transitions={"wealth": twin_next_wealth},
+                         constraints={"borrowing_constraint": borrowing_constraint}, functions=fr)
+    else:
+        working = ConsumptionSavingsRegime(
+            transition=twin_next_regime_from_working, states={"wealth": W}, actions=acts_w,
+            taste_shocks=ExtremeValueTasteShocks(), active=active,
+            state_transitions={"wealth": twin_next_wealth_from_savings},
+            functions={**fw, "savings": savings, "inverse_marginal_utility": twin_inverse_marginal_utility},
+            solver=solver, liquid=margin)
+        retired = ConsumptionSavingsRegime(
+            transition=twin_next_regime_from_retirement, states={"wealth": W},
+            actions={"consumption": Cg}, active=active,
+            state_transitions={"wealth": twin_next_wealth_from_savings},
+            functions={**fr, "savings": savings, "inverse_marginal_utility": twin_inverse_marginal_utility},
+            solver=solver, liquid=margin)
+    done_w = Regime(transition=None, states={"wealth": W},
+                    functions={"utility": twin_utility_done_from_working})
+    done_r = Regime(transition=None, states={"wealth": W}, functions={"utility": twin_utility_done_retired})
+    return Model(regimes={"working_life": working, "retirement": retired,
+                          "done_from_working": done_w, "done_retired": done_r},
+                 ages=AgeGrid(start=TWIN_MIN_AGE, stop=TWIN_MIN_AGE + TWIN_N_PERIODS - 1, step="Y"),
+                 regime_id_class=TwinRegimeId)
+
+
+def dcegm_twin_params(taste_shock_scale: float = 0.2) -> dict:
+    return {
+        "discount_factor": 0.95, "interest_rate": 0.05, "rho": 1.95, "delta": 0.35,
+        "constant": 0.75, "exp_coeff": 0.04, "exp_squared": -0.0002, "floor": 0.001,
+        "done_from_working": {"utility": {"taste_scale": taste_shock_scale}},
+        "working_life": {"taste_shocks": {"scale": taste_shock_scale}},
+        "retirement": {"next_wealth": {"labor_income": 0.0}},
+    }
+
+
+# -- DC-EGM retirement model with a log-normal income shock --------------------
+
+
+def next_wealth_stochastic(savings, labor_income, income, interest_rate):
+    """Savings grow at 1 + r; labor income scaled by the log-normal income level."""
     return (1 + interest_rate) * savings + labor_income * income
+
+
+def _nw_gs(wealth, consumption, labor_income, income, interest_rate):
+    return (1 + interest_rate) * (wealth - consumption) + labor_income * income
+
+
+def exp_savings_points(n_savings: int, savings_max: float, kappa: float = 3.0):
+    """Exponentially spaced savings nodes on [0, savings_max]:
+    `A_i = savings_max * (exp(kappa i/(n-1)) - 1) / (exp(kappa) - 1)`."""
+    i = np.arange(n_savings)
+    return savings_max * np.expm1(kappa * i / (n_savings - 1)) / np.expm1(kappa)
 
 
 def dcegm_retirement_model(
@@ -254,16 +460,15 @@ def dcegm_retirement_model(
     n_constrained_points: int = 20,
     grid_search: bool = False,
 ) -> Model:
-    """DC-EGM retirement model: log-normal income on GH nodes, EV1 taste shocks.
+    """DC-EGM retirement model: CRRA utility, work/retire choice with EV1 taste
+    shocks, absorbing retirement, a log-normal IID income level on Gauss-Hermite
+    nodes scaling the wage, exponentially spaced savings nodes.
 
-    Savings nodes are exponentially spaced on [0, savings_max]:
-    `A_i = savings_max * (exp(kappa i/(n-1)) - 1) / (exp(kappa) - 1)`.
     With `grid_search=True` the same economics is solved by grid search over
     `n_consumption` consumption points (the comparison answer).
     """
     last_age = n_periods - 1
-    i = np.arange(n_savings)
-    A = savings_max * np.expm1(kappa * i / (n_savings - 1)) / np.expm1(kappa)
+    A = exp_savings_points(n_savings, savings_max, kappa)
     sav_grid = IrregSpacedGrid(points=tuple(float(a) for a in A))
     W = LinSpacedGrid(start=wealth_range[0], stop=wealth_range[1], n_points=n_wealth)
     Cg = LinSpacedGrid(start=wealth_range[0], stop=wealth_range[1], n_points=n_consumption)
@@ -274,8 +479,8 @@ def dcegm_retirement_model(
     ts = ExtremeValueTasteShocks() if taste_shocks else None
     common = dict(states=states, active=lambda age, la=last_age: age < la)
     fw = {"utility": utility_working, "crra_utility": crra_utility, "labor_income": labor_income,
-          "is_working": is_working, "savings": savings}
-    fr = {"utility": utility_retirement, "crra_utility": crra_utility, "savings": savings}
+          "is_working": is_working}
+    fr = {"utility": utility_retirement, "crra_utility": crra_utility}
     if grid_search:
         working = Regime(actions={"labor_supply": DiscreteGrid(LaborSupply), "consumption": Cg},
                          state_transitions={"wealth": _nw_gs}, transition=next_regime_from_working,
@@ -286,20 +491,17 @@ def dcegm_retirement_model(
                          constraints={"borrowing_constraint": borrowing_constraint},
                          functions=fr, **common)
     else:
+        inv = {"savings": savings, "inverse_marginal_utility": inverse_marginal_utility_crra}
         working = ConsumptionSavingsRegime(
             actions={"labor_supply": DiscreteGrid(LaborSupply), "consumption": Cg},
-            state_transitions={"wealth": next_wealth_from_savings}, transition=next_regime_from_working,
-            functions=fw, solver=solver, liquid=margin, taste_shocks=ts, **common)
+            state_transitions={"wealth": next_wealth_stochastic}, transition=next_regime_from_working,
+            functions={**fw, **inv}, solver=solver, liquid=margin, taste_shocks=ts, **common)
         retired = ConsumptionSavingsRegime(
-            actions={"consumption": Cg}, state_transitions={"wealth": next_wealth_from_savings},
-            transition=next_regime_from_retirement, functions=fr, solver=solver, liquid=margin,
-            **common)
+            actions={"consumption": Cg}, state_transitions={"wealth": next_wealth_stochastic},
+            transition=next_regime_from_retirement, functions={**fr, **inv}, solver=solver,
+            liquid=margin, **common)
     return Model(regimes={"working_life": working, "retirement": retired, "dead": _dead()},
                  ages=AgeGrid(start=0, stop=last_age, step="Y"), regime_id_class=RegimeId)
-
-
-def _nw_gs(wealth, consumption, labor_income, income, interest_rate):
-    return (1 + interest_rate) * (wealth - consumption) + labor_income * income
 
 
 def dcegm_params(n_periods, *, discount_factor=0.96, crra=2.0, disutility_of_work=0.5,
