@@ -286,6 +286,113 @@ int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                      const LcmGsTask* d_tasks, int n_tasks, int tile,
                      size_t lds_bytes, int variant);
 
+/* ------------------------------------------------------------------------- */
+/* DC-EGM with the FUES upper envelope (all arithmetic FP64)                  */
+/* ------------------------------------------------------------------------- */
+
+#define LCM_EGM_MAX_TARGETS 4
+#define LCM_EGM_MAX_ROWS 8      /* discrete-action rows of one child block */
+#define LCM_EGM_MAX_CAND 4096   /* constrained + Euler candidates per column */
+#define LCM_TAPE_MAX_INS 15     /* instructions of one expression tape */
+
+/*
+ * One carry row of a child regime: the refined endogenous grid, value and
+ * marginal-utility rows (`EGMCarry`, src/_lcm/egm/carry.py:25-60), NaN-padded to
+ * `len`; *n_valid counts the non-NaN grid entries (written by the kernel that
+ * produced the row, or by the host for a terminal regime's closed-form rows,
+ * src/_lcm/egm/terminal.py:85-180).
+ */
+typedef struct LcmEgmRow {
+  const double* grid;
+  const double* value;
+  const double* marg;
+  const int* n_valid;
+  int len;
+  int pad;
+} LcmEgmRow;
+
+/*
+ * One (discrete state, discrete action) column of a DC-EGM regime at one
+ * period. Replaces the per-combo solve `_get_solve_one_combo`
+ * (src/_lcm/egm/step_core.py:130-312) with its continuation
+ * (src/_lcm/egm/continuation.py:323-500,959-1140,1353-1534), Euler inversion
+ * (src/_lcm/egm/euler.py:18-68), constrained candidates (step_core.py:443-514),
+ * FUES refinement (src/_lcm/egm/upper_envelope/fues.py:151-1189) and publish
+ * (step_core.py:517-616).
+ *
+ * Expression tapes (u_prog, i_prog: int16 programs, see pylcm_amd/_tape.py)
+ * evaluate the utility u(c) with u'(c) and the inverse marginal utility; u_c
+ * and i_c point at this column's constants. Per target t (regime reachable
+ * next period): kind 0 adds tprob * tscalar to the value only (stateless
+ * terminal); kind 1 reads K child nodes with weights tW[k], each a block of
+ * tN rows tRows[k*tN + d'] aggregated by logsum (tscale > 0) or hard max, at
+ * the next resources tR[j] with gradient tG[j] for savings node j.
+ * Outputs: the refined carry rows (n_pad each), counts[0] = kept envelope
+ * points (> n_pad: overflow, rows poisoned), counts[1] = valid grid entries,
+ * and the published V on the state grid W. cand is scratch of
+ * 4 * (n_con + n_s) doubles.
+ */
+typedef struct LcmEgmCol {
+  const double* A;
+  const double* W;
+  const short* u_prog;
+  const double* u_c;
+  const short* i_prog;
+  const double* i_c;
+  const double* tR[LCM_EGM_MAX_TARGETS];
+  const double* tG[LCM_EGM_MAX_TARGETS];
+  const double* tW[LCM_EGM_MAX_TARGETS];
+  const LcmEgmRow* tRows[LCM_EGM_MAX_TARGETS];
+  double tprob[LCM_EGM_MAX_TARGETS];
+  double tscalar[LCM_EGM_MAX_TARGETS];
+  double tscale[LCM_EGM_MAX_TARGETS];
+  int tK[LCM_EGM_MAX_TARGETS];
+  int tN[LCM_EGM_MAX_TARGETS];
+  int tkind[LCM_EGM_MAX_TARGETS];
+  double* out_grid;
+  double* out_policy;
+  double* out_value;
+  double* out_marg;
+  int* out_counts;
+  double* out_V;
+  double* cand;
+  double beta;
+  double ratio;       /* (1 / 1e-4) ** (1 / max(n_con - 1, 1)) */
+  double jump_thresh;
+  double w_max;       /* max over W */
+  int n_s;
+  int n_w;
+  int n_con;
+  int n_pad;
+  int n_scan;         /* FUES window; <= 0 scans exhaustively */
+  int n_targets;
+  int pad0;
+  int pad1;
+} LcmEgmCol;
+
+/* V of one discrete-state combo: logsum (scale > 0) or max over its n_d
+ * column rows (src/_lcm/egm/step.py:731-738), written in the output dtype. */
+typedef struct LcmEgmAgg {
+  const double* cols;  /* [n_d * n_w] */
+  void* out;           /* [n_w] */
+  double scale;
+  int n_d;
+  int n_w;
+} LcmEgmAgg;
+
+/* One launch solves n_cols columns (one workgroup each); lds_bytes from
+ * lcm_egm_lds_bytes(max candidates of any column). Columns of one launch must
+ * not read each other's outputs. */
+int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes);
+int lcm_egm_aggregate(lcm_ctx* ctx, int dtype, const LcmEgmAgg* d_aggs, int n_aggs, int max_w);
+size_t lcm_egm_lds_bytes(int n_cand);
+int lcm_abi_sizes_egm(size_t* row, size_t* col, size_t* agg);
+
+/* Evaluate an expression tape at n points for testing: out_v[i], out_d[i] =
+ * value and derivative at x[i] with constants c. */
+int lcm_tape_eval(lcm_ctx* ctx, const short* d_prog, const double* d_c, const double* d_x,
+                  double* d_v, double* d_d, int n);
+
 #ifdef __cplusplus
 }
 #endif

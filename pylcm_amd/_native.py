@@ -106,6 +106,37 @@ class LcmSimOut(C.Structure):
     _fields_ = [("regime", vp), ("action", vp), ("value", vp), ("w", vp), ("z", vp), ("sdd", vp)]
 
 
+EGM_MAX_TARGETS = 4
+EGM_MAX_ROWS = 8
+EGM_MAX_CAND = 4096
+
+
+class LcmEgmRow(C.Structure):
+    _fields_ = [("grid", vp), ("value", vp), ("marg", vp), ("n_valid", vp), ("len", C.c_int),
+                ("pad", C.c_int)]
+
+
+class LcmEgmCol(C.Structure):
+    _fields_ = [
+        ("A", vp), ("W", vp), ("u_prog", vp), ("u_c", vp), ("i_prog", vp), ("i_c", vp),
+        ("tR", vp * EGM_MAX_TARGETS), ("tG", vp * EGM_MAX_TARGETS), ("tW", vp * EGM_MAX_TARGETS),
+        ("tRows", vp * EGM_MAX_TARGETS),
+        ("tprob", C.c_double * EGM_MAX_TARGETS), ("tscalar", C.c_double * EGM_MAX_TARGETS),
+        ("tscale", C.c_double * EGM_MAX_TARGETS),
+        ("tK", C.c_int * EGM_MAX_TARGETS), ("tN", C.c_int * EGM_MAX_TARGETS),
+        ("tkind", C.c_int * EGM_MAX_TARGETS),
+        ("out_grid", vp), ("out_policy", vp), ("out_value", vp), ("out_marg", vp),
+        ("out_counts", vp), ("out_V", vp), ("cand", vp),
+        ("beta", C.c_double), ("ratio", C.c_double), ("jump_thresh", C.c_double), ("w_max", C.c_double),
+        ("n_s", C.c_int), ("n_w", C.c_int), ("n_con", C.c_int), ("n_pad", C.c_int), ("n_scan", C.c_int),
+        ("n_targets", C.c_int), ("pad0", C.c_int), ("pad1", C.c_int),
+    ]
+
+
+class LcmEgmAgg(C.Structure):
+    _fields_ = [("cols", vp), ("out", vp), ("scale", C.c_double), ("n_d", C.c_int), ("n_w", C.c_int)]
+
+
 _SIGS = {
     "lcm_init": ([C.c_int], C.c_int),
     "lcm_last_error": ([], C.c_char_p),
@@ -140,6 +171,11 @@ _SIGS = {
     "lcm_sim_period_ex": ([vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_uint64,
                            C.c_size_t, vp, vp], C.c_int),
     "lcm_sim_lds_bytes": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
+    "lcm_egm_launch": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
+    "lcm_egm_aggregate": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),
+    "lcm_egm_lds_bytes": ([C.c_int], C.c_size_t),
+    "lcm_abi_sizes_egm": ([C.POINTER(C.c_size_t)] * 3, C.c_int),
+    "lcm_tape_eval": ([vp, vp, vp, vp, vp, vp, C.c_int], C.c_int),
     "lcm_philox": ([vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32], C.c_int),
 }
 
