@@ -294,6 +294,7 @@ int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
 #define LCM_EGM_MAX_ROWS 8      /* discrete-action rows of one child block */
 #define LCM_EGM_MAX_CAND 4096   /* constrained + Euler candidates per column */
 #define LCM_TAPE_MAX_INS 15     /* instructions of one expression tape */
+#define LCM_EGM_MAX_ROWS_PER_COL 64  /* child rows (all targets and nodes) one column reads */
 
 /*
  * One carry row of a child regime: the refined endogenous grid, value and
@@ -330,7 +331,7 @@ typedef struct LcmEgmRow {
  * Outputs: the refined carry rows (n_pad each), counts[0] = kept envelope
  * points (> n_pad: overflow, rows poisoned), counts[1] = valid grid entries,
  * and the published V on the state grid W. cand is scratch of
- * 4 * (n_con + n_s) doubles.
+ * 4 * (n_con + n_s) + 1 doubles.
  */
 typedef struct LcmEgmCol {
   const double* A;
@@ -380,10 +381,14 @@ typedef struct LcmEgmAgg {
   int n_w;
 } LcmEgmAgg;
 
-/* One launch solves n_cols columns (one workgroup each); lds_bytes from
- * lcm_egm_lds_bytes(max candidates of any column). Columns of one launch must
- * not read each other's outputs. */
-int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes);
+/* One call solves n_cols columns: a continuation/Euler kernel over blocks of
+ * 64 savings nodes, then one workgroup per column for the envelope and the
+ * publish. lds_bytes from lcm_egm_lds_bytes(max candidates of any column);
+ * max_rows = the most child rows any column reads (sum over its targets of
+ * nodes x rows, <= LCM_EGM_MAX_ROWS_PER_COL); max_s = the most savings nodes.
+ * Columns of one call must not read each other's outputs. */
+int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
+                   int max_s);
 int lcm_egm_aggregate(lcm_ctx* ctx, int dtype, const LcmEgmAgg* d_aggs, int n_aggs, int max_w);
 size_t lcm_egm_lds_bytes(int n_cand);
 int lcm_abi_sizes_egm(size_t* row, size_t* col, size_t* agg);

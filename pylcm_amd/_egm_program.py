@@ -277,6 +277,11 @@ class EgmProgram:
                 raise UnsupportedModelError(f"target {x!r}: more than {nat.EGM_MAX_ROWS} discrete-action rows")
             targets.append({"kind": 1, "prob": pr, "W_off": ar.add(Wt, dedupe=False), "K": Wt.shape[1],
                             "N": info["N"], "scale": info["scale"], "child": (t + 1, x)})
+        n_rows = sum(tg["K"] * tg["N"] for tg in targets if tg["kind"] == 1)
+        if n_rows > nat.EGM_MAX_ROWS_PER_COL:
+            raise UnsupportedModelError(
+                f"regime {r!r}: its columns read {n_rows} child rows; the device limit is "
+                f"{nat.EGM_MAX_ROWS_PER_COL}")
 
         # outputs
         slot = 8 * n_pad
@@ -300,7 +305,7 @@ class EgmProgram:
                     ic_n=ic.shape[1], R=R_off, G=G_off, n_s=n_s, n_w=n_w, n_con=n_con, n_pad=n_pad,
                     n_cols=n_cols, beta=beta, ratio=ratio, jump=float(solver.envelope.jump_thresh),
                     n_scan=0 if n_scan is None else int(n_scan), w_max=float(np.max(W)),
-                    targets=targets, out=self.col_out[(t, r)])
+                    targets=targets, out=self.col_out[(t, r)], n_rows=n_rows)
         self._col_specs.setdefault(t, []).append(spec)
         self._agg_specs.setdefault(t, []).append((t, r, scale))
         self.max_cand = max(self.max_cand, n_cand)
@@ -314,7 +319,7 @@ class EgmProgram:
             if info["kind"] == "rows":
                 self._rowtab[key] = ar.reserve(len(info["rows"]) * ROW.itemsize)
         self._period = {}
-        cand_per_col = 4 * self.max_cand * 8
+        cand_per_col = ((4 * self.max_cand + 1) * 8 + 255) // 256 * 256
         max_cols = max((sum(s["n_cols"] for s in specs) for specs in self._col_specs.values()), default=0)
         self.cand_off = ar.reserve(max(max_cols, 1) * max(cand_per_col, 8))
         for t, specs in self._col_specs.items():
@@ -391,7 +396,8 @@ class EgmProgram:
                     ai += 1
                 max_w = max(max_w, o["n_w"])
             put(agg_off, aggs)
-            self.lds_by_t[t] = (int(nat.lib().lcm_egm_lds_bytes(max_cand)), max_w)
+            self.lds_by_t[t] = (int(nat.lib().lcm_egm_lds_bytes(max_cand)), max_w,
+                                max(sp["n_rows"] for sp in specs), max(sp["n_s"] for sp in specs))
         nat.h2d(self.ctx, self.buf.ptr, img[: ar.size])
         self.ctx.sync()
 
@@ -402,9 +408,9 @@ class EgmProgram:
         base = self.buf.ptr
         for t in sorted(self._period, reverse=True):
             col_off, n_cols, agg_off, n_aggs = self._period[t]
-            lds, max_w = self.lds_by_t[t]
-            nat._check(L.lcm_egm_launch(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds)),
-                       f"lcm_egm_launch (period {t})")
+            lds, max_w, max_rows, max_s = self.lds_by_t[t]
+            nat._check(L.lcm_egm_launch(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
+                                        max_rows, max_s), f"lcm_egm_launch (period {t})")
             nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
                        f"lcm_egm_aggregate (period {t})")
         if sync:

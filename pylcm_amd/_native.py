@@ -109,6 +109,7 @@ class LcmSimOut(C.Structure):
 EGM_MAX_TARGETS = 4
 EGM_MAX_ROWS = 8
 EGM_MAX_CAND = 4096
+EGM_MAX_ROWS_PER_COL = 64
 
 
 class LcmEgmRow(C.Structure):
@@ -171,7 +172,7 @@ _SIGS = {
     "lcm_sim_period_ex": ([vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_uint64,
                            C.c_size_t, vp, vp], C.c_int),
     "lcm_sim_lds_bytes": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
-    "lcm_egm_launch": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
+    "lcm_egm_launch": ([vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int], C.c_int),
     "lcm_egm_aggregate": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),
     "lcm_egm_lds_bytes": ([C.c_int], C.c_size_t),
     "lcm_abi_sizes_egm": ([C.POINTER(C.c_size_t)] * 3, C.c_int),

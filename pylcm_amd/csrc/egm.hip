@@ -1,8 +1,9 @@
 // DC-EGM with the FUES upper envelope on gfx950: one workgroup (4 waves) per
 // (discrete state, discrete action) column, all arithmetic in FP64.
 //
-// Phases of egm_column_kernel, separated by workgroup barriers:
-//   1. continuation + Euler inversion, one savings node per lane
+// Phases, separated by workgroup barriers (phase 1 is its own kernel,
+// egm_euler_kernel, spread over blocks of 64 savings nodes):
+//   1. continuation + Euler inversion, one savings node per thread
 //      (src/_lcm/egm/continuation.py:323-500,959-1140,1353-1534;
 //       src/_lcm/egm/euler.py:18-68; src/_lcm/egm/step_core.py:397-409);
 //   2. closed-form constrained candidates and the -inf -> NaN masking
@@ -143,27 +144,85 @@ __device__ __forceinline__ double between(double xq, double xl, double xu, doubl
   return lin + (ok ? corr : 0.0);
 }
 
-// x NaN -> NaN; valid 0 -> NaN; valid 1 -> fp[0]; else the bracket of
-// searchsorted(side="right") clipped to [1, valid - 1].
-__device__ double read_row(double x, const double* __restrict__ xp, const double* __restrict__ fp,
-                           const double* __restrict__ sp, int valid) {
-  if (isnan_(x) || valid == 0) return qnan();
-  if (valid == 1) return fp[0];
-  int lo = 0, hi = valid;  // first index with xp > x
-  while (lo < hi) {
-    const int m = (lo + hi) >> 1;
-    if (xp[m] <= x) lo = m + 1;
-    else hi = m;
+// Number of entries of xp[0, valid) that are <= x (searchsorted side="right"),
+// found by galloping from a previous answer `hint` (any value in [0, valid]):
+// monotone query sequences resolve in a couple of loads.
+__device__ int count_le(const double* __restrict__ xp, int valid, double x, int hint) {
+  int L = 0, H = valid;
+  const int h = hint < 0 ? 0 : (hint > valid ? valid : hint);
+  if (h < valid && xp[h] <= x) {  // the answer is above h: gallop right
+    L = h + 1;
+    for (int step = 1; L < H; step <<= 1) {
+      const int m = min(L + step - 1, H - 1);
+      if (xp[m] <= x) {
+        L = m + 1;
+      } else {
+        H = m;
+        break;
+      }
+    }
+  } else {  // the answer is at most h: gallop left
+    H = h;
+    if (h > 0) {
+      if (xp[h - 1] <= x) return h;
+      H = h - 1;
+    }
+    for (int step = 1; L < H; step <<= 1) {
+      const int m = max(H - step, L);
+      if (xp[m] > x) {
+        H = m;
+      } else {
+        L = m + 1;
+        break;
+      }
+    }
   }
-  int up = lo < 1 ? 1 : (lo > valid - 1 ? valid - 1 : lo);
+  while (L < H) {
+    const int m = (L + H) >> 1;
+    if (xp[m] <= x) L = m + 1;
+    else H = m;
+  }
+  return L;
+}
+
+// Value (Hermite with the marginal row as node slopes) and marginal (linear)
+// reads of one carry row at x, sharing one bracket; `hint` is updated.
+__device__ __forceinline__ void read_pair(double x, const LcmEgmRow& rw, int valid, int& hint, double& val,
+                                          double& mar) {
+  if (isnan_(x) || valid == 0) {
+    val = qnan();
+    mar = qnan();
+    return;
+  }
+  if (valid == 1) {
+    val = rw.value[0];
+    mar = rw.marg[0];
+    return;
+  }
+  const int u = count_le(rw.grid, valid, x, hint);
+  hint = u;
+  const int up = u < 1 ? 1 : (u > valid - 1 ? valid - 1 : u);
   const int dn = up - 1;
-  if (sp) return between(x, xp[dn], xp[up], fp[dn], fp[up], true, sp[dn], sp[up]);
-  return between(x, xp[dn], xp[up], fp[dn], fp[up], false, 0.0, 0.0);
+  const double xl = rw.grid[dn], xu = rw.grid[up];
+  const double sl = rw.marg[dn], su = rw.marg[up];
+  val = between(x, xl, xu, rw.value[dn], rw.value[up], true, sl, su);
+  mar = between(x, xl, xu, sl, su, false, 0.0, 0.0);
 }
 
 // ---------------------------------------------------------------------------
 // FUES helpers (fues.py:1082-1189)
 // ---------------------------------------------------------------------------
+
+// fabs(a / b) > th, as the rounded quotient compares, deciding by a product
+// when the ratio is clearly away from th (b != 0).
+__device__ __forceinline__ bool abs_ratio_gt(double a, double b, double th) {
+  const double aa = fabs(a), tb = th * fabs(b);
+  if (tb > 1e-290 && isfin(tb) && isfin(aa)) {
+    if (aa > tb * (1.0 + 1e-14)) return true;
+    if (aa < tb * (1.0 - 1e-14)) return false;
+  }
+  return fabs(a / b) > th;
+}
 
 __device__ __forceinline__ double slope(double xa, double ya, double xb, double yb) {
   const double dx = xb - xa;
@@ -264,6 +323,133 @@ __device__ int find_backward(const LdsRows& r, int idx, int n_scan, double th, d
 // the column kernel
 // ---------------------------------------------------------------------------
 
+// Aggregates one child block of N rows at one savings node: the logsum with
+// its softmax-weighted marginal (scale > 0) or the hard max with the first
+// argmax's marginal (continuation.py:1317-1351, logsum.py:32-64).
+__device__ __forceinline__ void aggregate_rows(const double* vals, const double* mars, int stride, int N,
+                                               double scale, double& sv, double& sm) {
+  double m = ninf();
+  bool any_nan = false;
+  int arg = -1;
+  for (int d = 0; d < N; ++d) {
+    const double val = vals[d * stride];
+    if (isnan_(val)) {
+      if (!any_nan) arg = d;
+      any_nan = true;
+    } else if (!any_nan && (arg < 0 || val > m)) {
+      m = val;
+      arg = d;
+    }
+  }
+  if (arg < 0) arg = 0;
+  if (any_nan) m = qnan();
+  if (scale > 0.0) {
+    const double fm = isneginf_(m) ? 0.0 : m;
+    double sum = 0.0, shmax = ninf();
+    bool all_masked = true;
+    for (int d = 0; d < N; ++d) {
+      const double v = vals[d * stride];
+      const double sh = (v - fm) / scale;
+      sum = sum + exp(sh);
+      shmax = (isnan_(sh) || isnan_(shmax)) ? qnan() : (sh > shmax ? sh : shmax);
+      all_masked = all_masked && isneginf_(v);
+    }
+    sv = fm + scale * log(sum);
+    double esum = 0.0;
+    for (int d = 0; d < N; ++d) esum = esum + exp((vals[d * stride] - fm) / scale - shmax);
+    sm = 0.0;
+    for (int d = 0; d < N; ++d) {
+      const double pd = all_masked ? 0.0 : exp((vals[d * stride] - fm) / scale - shmax) / esum;
+      sm = sm + pd * mars[d * stride];
+    }
+  } else {
+    sv = m;
+    sm = 0.0;
+    for (int d = 0; d < N; ++d) sm = sm + (d == arg ? 1.0 : 0.0) * mars[d * stride];
+  }
+}
+
+constexpr int kNodes = 64;  // savings nodes per block of the Euler kernel
+
+// Phase 1: continuation and Euler inversion. Block (bx, col) takes savings
+// nodes [64 bx, 64 bx + 64) of column col; its 256 threads read every (child
+// row, node) pair in parallel into LDS, then one thread per node aggregates
+// over the child's choice rows, nodes and target regimes and inverts the
+// Euler equation. Writes candidate slots n_con + j and, for node 0, E[V'] at
+// the savings limit (cand[4n]).
+__global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __restrict__ cols) {
+  const LcmEgmCol& C = cols[blockIdx.y];
+  const int tid = threadIdx.x;
+  const int n_s = C.n_s, n_con = C.n_con, n = n_s + n_con;
+  const int j0 = blockIdx.x * kNodes;
+  if (j0 >= n_s) return;
+  const double beta = C.beta;
+  const double eps = 2.220446049250313e-16;
+  int Q = 0;
+  for (int t = 0; t < C.n_targets; ++t)
+    if (C.tkind[t] != 0) Q += C.tK[t] * C.tN[t];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double* vals = reinterpret_cast<double*>(lds_raw);  // [Q][kNodes]
+  double* mars = vals + (size_t)Q * kNodes;
+  for (int pq = tid; pq < Q * kNodes; pq += kThreads) {
+    const int q = pq / kNodes, jj = pq - q * kNodes, j = j0 + jj;
+    if (j >= n_s) continue;
+    int t = 0, base = 0;
+    for (; t < C.n_targets; ++t) {
+      if (C.tkind[t] == 0) continue;
+      const int rows_t = C.tK[t] * C.tN[t];
+      if (q < base + rows_t) break;
+      base += rows_t;
+    }
+    const LcmEgmRow& rw = C.tRows[t][q - base];
+    const int valid = *rw.n_valid;
+    int hint = valid >> 1;
+    double val, mar;
+    read_pair(C.tR[t][j], rw, valid, hint, val, mar);
+    mar = mar * C.tG[t][j];
+    if (isneginf_(val)) mar = 0.0;
+    vals[pq] = val;
+    mars[pq] = mar;
+  }
+  __syncthreads();
+  const int j = j0 + tid;
+  if (tid >= kNodes || j >= n_s) return;
+  double EV = 0.0, EMU = 0.0;
+  int row0 = 0;
+  for (int t = 0; t < C.n_targets; ++t) {
+    const double pr = C.tprob[t];
+    if (C.tkind[t] == 0) {
+      EV = EV + (pr > 0.0 ? pr * C.tscalar[t] : pr * 0.0);
+      continue;
+    }
+    const int K = C.tK[t], N = C.tN[t];
+    double vs = 0.0, ms = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const int off = (row0 + k * N) * kNodes + tid;
+      double sv, sm;
+      aggregate_rows(vals + off, mars + off, kNodes, N, C.tscale[t], sv, sm);
+      const double w = C.tW[t][k];
+      vs = vs + zero_safe(w, sv);
+      ms = ms + zero_safe(w, sm);
+    }
+    row0 += K * N;
+    EV = EV + (pr > 0.0 ? pr * vs : pr * 0.0);
+    EMU = EMU + (pr > 0.0 ? pr * ms : pr * 0.0);
+  }
+  const double disc = beta * EMU;
+  const double target = disc < 0.0 ? qnan() : nmax(disc, eps);
+  const double c = tape_eval(C.i_prog, C.i_c, target, nullptr);
+  const double Aj = C.A[j];
+  const double val = tape_eval(C.u_prog, C.u_c, c, nullptr) + beta * EV;
+  double* cg = C.cand;
+  cg[n_con + j] = Aj + c;
+  cg[n + n_con + j] = c;
+  cg[2 * n + n_con + j] = val;
+  cg[3 * n + n_con + j] = Aj;
+  if (j == 0) cg[4 * n] = EV;
+}
+
+// Phases 2-6 of one column (after egm_euler_kernel).
 __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* __restrict__ cols) {
   const LcmEgmCol& C = cols[blockIdx.x];
   const int tid = threadIdx.x;
@@ -275,104 +461,13 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
   const double beta = C.beta;
   const double eps = 2.220446049250313e-16;
 
-  __shared__ double sh_ev0;
   __shared__ int sh_live;
   __shared__ int sh_nkept;
   if (tid == 0) sh_live = 0;
-
-  // ---- 1. continuation and Euler inversion per savings node ----------------
-  for (int j = tid; j < n_s; j += kThreads) {
-    double EV = 0.0, EMU = 0.0;
-    for (int t = 0; t < C.n_targets; ++t) {
-      const double pr = C.tprob[t];
-      if (C.tkind[t] == 0) {
-        EV = EV + (pr > 0.0 ? pr * C.tscalar[t] : pr * 0.0);
-        continue;
-      }
-      const double Rq = C.tR[t][j];
-      const double gR = C.tG[t][j];
-      const int K = C.tK[t], N = C.tN[t];
-      const double scale = C.tscale[t];
-      double vs = 0.0, ms = 0.0;
-      for (int k = 0; k < K; ++k) {
-        const LcmEgmRow* rows = C.tRows[t] + (size_t)k * N;
-        double vals[LCM_EGM_MAX_ROWS], mars[LCM_EGM_MAX_ROWS];
-        double m = ninf();
-        bool any_nan = false;
-        int arg = -1;
-#pragma unroll
-        for (int d = 0; d < LCM_EGM_MAX_ROWS; ++d) {
-          if (d < N) {
-            const LcmEgmRow& rw = rows[d];
-            const int valid = *rw.n_valid;
-            const double val = read_row(Rq, rw.grid, rw.value, rw.marg, valid);
-            double mar = read_row(Rq, rw.grid, rw.marg, nullptr, valid) * gR;
-            if (isneginf_(val)) mar = 0.0;
-            vals[d] = val;
-            mars[d] = mar;
-            if (isnan_(val)) {
-              if (!any_nan) arg = d;
-              any_nan = true;
-            } else if (!any_nan && (arg < 0 || val > m)) {
-              m = val;
-              arg = d;
-            }
-          }
-        }
-        if (arg < 0) arg = 0;
-        if (any_nan) m = qnan();
-        double sv, sm;
-        if (scale > 0.0) {
-          const double fm = isneginf_(m) ? 0.0 : m;
-          double sum = 0.0, shmax = ninf();
-          bool all_masked = true;
-#pragma unroll
-          for (int d = 0; d < LCM_EGM_MAX_ROWS; ++d) {
-            if (d < N) {
-              const double sh = (vals[d] - fm) / scale;
-              sum = sum + exp(sh);
-              shmax = (isnan_(sh) || isnan_(shmax)) ? qnan() : (sh > shmax ? sh : shmax);
-              all_masked = all_masked && isneginf_(vals[d]);
-            }
-          }
-          sv = fm + scale * log(sum);
-          double esum = 0.0;
-#pragma unroll
-          for (int d = 0; d < LCM_EGM_MAX_ROWS; ++d)
-            if (d < N) esum = esum + exp((vals[d] - fm) / scale - shmax);
-          sm = 0.0;
-#pragma unroll
-          for (int d = 0; d < LCM_EGM_MAX_ROWS; ++d) {
-            if (d < N) {
-              const double pd = all_masked ? 0.0 : exp((vals[d] - fm) / scale - shmax) / esum;
-              sm = sm + pd * mars[d];
-            }
-          }
-        } else {
-          sv = m;
-          sm = 0.0;
-#pragma unroll
-          for (int d = 0; d < LCM_EGM_MAX_ROWS; ++d)
-            if (d < N) sm = sm + (d == arg ? 1.0 : 0.0) * mars[d];
-        }
-        const double w = C.tW[t][k];
-        vs = vs + zero_safe(w, sv);
-        ms = ms + zero_safe(w, sm);
-      }
-      EV = EV + (pr > 0.0 ? pr * vs : pr * 0.0);
-      EMU = EMU + (pr > 0.0 ? pr * ms : pr * 0.0);
-    }
-    const double disc = beta * EMU;
-    const double target = disc < 0.0 ? qnan() : nmax(disc, eps);
-    const double c = tape_eval(C.i_prog, C.i_c, target, nullptr);
-    const double Aj = C.A[j];
-    const double val = tape_eval(C.u_prog, C.u_c, c, nullptr) + beta * EV;
-    cg[n_con + j] = Aj + c;
-    cp[n_con + j] = c;
-    cv[n_con + j] = val;
-    cs[n_con + j] = Aj;
-    if (j == 0) sh_ev0 = EV;
-  }
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  int P = kWave;
+  while (P < n) P <<= 1;
+  const double sh_ev0 = cg[4 * n];
   __syncthreads();
 
   // ---- 2. constrained candidates, dead masking ------------------------------
@@ -411,22 +506,26 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
   __syncthreads();
 
   // ---- 3. sort in LDS ---------------------------------------------------------
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  int P = kWave;
-  while (P < n) P <<= 1;
   LdsRows r;
   r.g = reinterpret_cast<double*>(lds_raw);
   r.p = r.g + P;
   r.v = r.p + P;
   r.s = r.v + P;
   r.idx = reinterpret_cast<int*>(r.s + P);
+  // candidates usually arrive sorted (the endogenous grid rises with the
+  // savings node away from kinks): skip the network when they do
+  int unsorted = 0;
+  for (int i = tid; i + 1 < n; i += kThreads) {
+    const double a = cg[i], b = cg[i + 1];
+    if (!sort_before(isnan_(a) ? pinf() : a, i, isnan_(b) ? pinf() : b, i + 1, n, cp, cv)) unsorted = 1;
+  }
   for (int i = tid; i < P; i += kThreads) {
     const double g = i < n ? cg[i] : pinf();
     r.g[i] = isnan_(g) ? pinf() : g;
     r.idx[i] = i;
   }
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
+  const bool need_sort = __syncthreads_or(unsorted) != 0;
+  for (int k = 2; need_sort && k <= P; k <<= 1) {
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
       for (int i = tid; i < P; i += kThreads) {
         const int l = i ^ jj;
@@ -499,31 +598,108 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
   __syncthreads();
 
   // ---- 5. FUES scan on wave 0 -------------------------------------------------
+  // simple[i]: with k = candidate i-2 and j = candidate i-1 in the carry, step i
+  // is a plain accept (no segment switch, not dropped). The scan jumps over runs
+  // of such steps, found 64 at a time, whenever the carry is in that state.
+  const double th = C.jump_thresh;
+  int* simple = r.idx;
+  for (int i = tid; i < n; i += kThreads) {
+    bool ok = false;
+    if (i >= 2) {
+      const double Gi = r.g[i], Pi = r.p[i], Vi = r.v[i], Si = r.s[i];
+      const double Gj = r.g[i - 1], Pj = r.p[i - 1], Vj = r.v[i - 1], Sj = r.s[i - 1];
+      const bool valid = !isnan_(Gi) && !isnan_(Vi);
+      const bool sw = (Gj == Gi) ? (Sj != Si) : abs_ratio_gt((Gi - Pi) - (Gj - Pj), Gi - Gj, th);
+      bool dropped = !valid || Vi < Vj;
+      if (!dropped && Si < Sj) dropped = slope(Gj, Vj, Gi, Vi) < slope(r.g[i - 2], r.v[i - 2], Gj, Vj);
+      ok = !sw && !dropped;
+    }
+    simple[i] = ok ? 1 : 0;
+  }
+  __syncthreads();
   const int n_pad = C.n_pad;
   double* og = C.out_grid;
   double* op = C.out_policy;
   double* ov = C.out_value;
   if (tid < kWave) {
     const int lane = tid;
-    const double th = C.jump_thresh;
     const int n_scan = C.n_scan > 0 ? C.n_scan : max(n - 1, 1);
     double Gk = r.g[0], Pk = r.p[0], Vk = r.v[0];
     double Gj = Gk, Pj = Pk, Vj = Vk, Sj = r.s[0];
     int total = 0;
+    int jidx = 0;        // candidate held in j
+    bool kcanon = false;  // k holds candidate jidx - 1
+    double nG = n > 1 ? r.g[1] : 0.0, nP = n > 1 ? r.p[1] : 0.0, nV = n > 1 ? r.v[1] : 0.0;
+    double nS = n > 1 ? r.s[1] : 0.0;
     for (int idx = 1; idx < n; ++idx) {
-      const double Gi = r.g[idx], Pi = r.p[idx], Vi = r.v[idx], Si = r.s[idx];
+      if (kcanon && jidx == idx - 1) {
+        int end = n;
+        for (int base = idx; base < n; base += kWave) {
+          const int pos = base + lane;
+          const unsigned long long stop = __ballot(pos < n && simple[pos] == 0);
+          if (stop) {
+            end = base + __ffsll((long long)stop) - 1;
+            break;
+          }
+        }
+        const int cnt = end - idx;
+        if (cnt > 0) {
+          // steps idx .. end-1 each finalise the previous candidate
+          for (int l = lane; l < cnt; l += kWave) {
+            const int o = total + l;
+            if (o < n_pad) {
+              og[o] = r.g[idx - 1 + l];
+              op[o] = r.p[idx - 1 + l];
+              ov[o] = r.v[idx - 1 + l];
+            }
+          }
+          total += cnt;
+          Gk = r.g[end - 2]; Pk = r.p[end - 2]; Vk = r.v[end - 2];
+          Gj = r.g[end - 1]; Pj = r.p[end - 1]; Vj = r.v[end - 1]; Sj = r.s[end - 1];
+          jidx = end - 1;
+          idx = end;
+          if (idx >= n) break;
+          nG = r.g[idx]; nP = r.p[idx]; nV = r.v[idx]; nS = r.s[idx];
+        }
+      }
+      const double Gi = nG, Pi = nP, Vi = nV, Si = nS;
+      if (idx + 1 < n) {  // next candidate's loads overlap this step's arithmetic
+        nG = r.g[idx + 1];
+        nP = r.p[idx + 1];
+        nV = r.v[idx + 1];
+        nS = r.s[idx + 1];
+      }
       const bool valid = !isnan_(Gi) && !isnan_(Vi);
-      const bool switches = (Gj == Gi) ? (Sj != Si) : (fabs(slope(Gj, Gj - Pj, Gi, Gi - Pi)) > th);
-      const bool node_cross = valid && switches && Gi == Gj && Vi == Vj;
+      const bool switches = (Gj == Gi) ? (Sj != Si) : abs_ratio_gt((Gi - Pi) - (Gj - Pj), Gi - Gj, th);
+      const bool sav_dec = Si < Sj;
+      if (!switches) {
+        // Without a segment switch there is no node crossing, no dominated j and
+        // no crossing to insert: the step either drops i or finalises j.
+        bool dropped = !valid || Vi < Vj;
+        if (!dropped && sav_dec) dropped = slope(Gj, Vj, Gi, Vi) < slope(Gk, Vk, Gj, Vj);
+        if (!dropped) {
+          if (lane == 0 && total < n_pad) {
+            og[total] = Gj;
+            op[total] = Pj;
+            ov[total] = Vj;
+          }
+          total += 1;
+          Gk = Gj; Pk = Pj; Vk = Vj;
+          Gj = Gi; Pj = Pi; Vj = Vi; Sj = Si;
+          kcanon = jidx == idx - 1;
+          jidx = idx;
+        }
+        continue;
+      }
       const double secant = slope(Gj, Vj, Gi, Vi);
       const double grad_before = slope(Gk, Vk, Gj, Vj);
+      const bool node_cross = valid && switches && Gi == Gj && Vi == Vj;
       int pj_pos, pf_pos;
       find_two_forward(r, n, idx, n_scan, th, Gj, Pj, Gi, Pi, pj_pos, pf_pos);
       const bool jf = pj_pos >= 0;
       const double jg = jf ? r.g[pj_pos] : 0.0, jp = jf ? r.p[pj_pos] : 0.0, jv = jf ? r.v[pj_pos] : 0.0;
       const double sec_i_jseg = slope(Gi, Vi, jg, jv);
       const bool below_j = switches && jf && ((Gi == jg) ? (Vi < jv) : (secant < sec_i_jseg));
-      const bool sav_dec = Si < Sj;
       const bool dropped = (!valid || (Vi < Vj && !switches) || (sav_dec && secant < grad_before) || below_j) &&
                            !node_cross;
       int pb_pos = -1;
@@ -589,6 +765,8 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
       total += cnt;
       if (!dropped) {
         const bool keeps_k = j_dom && !kink6;
+        kcanon = !keeps_k && !kink5 && !kink6 && !on_j && jidx == idx - 1;
+        jidx = idx;
         if (!keeps_k) {
           if (kink5) {
             Gk = kg5; Pk = kpr5; Vk = kv5;
@@ -647,9 +825,22 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
   }
   __syncthreads();
   const double g0 = og[0];
-  for (int w = tid; w < C.n_w; w += kThreads) {
+  const int w_per = (C.n_w + kThreads - 1) / kThreads;
+  const int w_end = min(C.n_w, (tid + 1) * w_per);
+  int w_hint = 0;
+  for (int w = tid * w_per; w < w_end; ++w) {
     const double x = C.W[w];
-    const double vint = read_row(x, og, ov, om, n_valid);
+    double vint;
+    if (isnan_(x) || n_valid == 0) {
+      vint = qnan();
+    } else if (n_valid == 1) {
+      vint = ov[0];
+    } else {
+      const int u = count_le(og, n_valid, x, w_hint);
+      w_hint = u;
+      const int up = u < 1 ? 1 : (u > n_valid - 1 ? n_valid - 1 : u);
+      vint = between(x, og[up - 1], og[up], ov[up - 1], ov[up], true, om[up - 1], om[up]);
+    }
     const double cf = x - limit;
     const double tiny = 2.2250738585072014e-308;
     const double vcon = cf > 0.0 ? tape_eval(C.u_prog, C.u_c, fmax(cf, tiny), nullptr) + dev0 : ninf();
@@ -699,7 +890,8 @@ extern "C" size_t lcm_egm_lds_bytes(int n_cand) {
   return (size_t)P * (4 * sizeof(double) + sizeof(int));
 }
 
-extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes) {
+extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
+                              int max_s) {
   if (!ctx || (!d_cols && n_cols)) return fail(LCM_E_ARG, "lcm_egm_launch: null argument");
   if (n_cols <= 0) return LCM_OK;
   if (lds_bytes > lcm_egm_lds_bytes(LCM_EGM_MAX_CAND))
@@ -709,6 +901,12 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
   LCM_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   if (lds_bytes + 64 > (size_t)max_lds)
     return fail(LCM_E_ARG, "lcm_egm_launch: the candidate rows do not fit in this device's LDS");
+  if (max_rows < 0 || max_rows > LCM_EGM_MAX_ROWS_PER_COL || max_s < 1 || max_s > LCM_EGM_MAX_CAND)
+    return fail(LCM_E_ARG, "lcm_egm_launch: row or node count out of range");
+  const size_t lds_euler = (size_t)max_rows * egm::kNodes * 2 * sizeof(double);
+  hipLaunchKernelGGL(egm::egm_euler_kernel, dim3((max_s + egm::kNodes - 1) / egm::kNodes, n_cols),
+                     dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
+  LCM_HIP(hipGetLastError());
   hipLaunchKernelGGL(egm::egm_column_kernel, dim3(n_cols), dim3(egm::kThreads), lds_bytes, ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
   return LCM_OK;
