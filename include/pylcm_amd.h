@@ -236,6 +236,10 @@ int lcm_event_create(void** ev);
 int lcm_event_destroy(void* ev);
 int lcm_event_record(lcm_ctx* ctx, void* ev);
 int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms);
+/* 1 when the work before the event's record has completed, 0 if not yet
+ * (a non-blocking poll, for the serving loop's retirements). */
+int lcm_event_query(void* ev);
+int lcm_event_synchronize(void* ev);
 
 /* Plain device-to-device copy kernel of `bytes` (a multiple of 16) on ctx's
  * stream. No counterpart in the reference: a kernel of known traffic whose

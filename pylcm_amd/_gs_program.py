@@ -180,17 +180,20 @@ class GsProgram:
         self._init([(model, p) for p in params_list], dtype, tile=tile, ctx=ctx, lowered=lowered)
 
     @classmethod
-    def of(cls, problems, dtype, *, tile=None, ctx=None, lowered=None):
+    def of(cls, problems, dtype, *, tile=None, ctx=None, lowered=None, defer=False):
+        """With `defer=True` only the host image is built; `materialize(ptr)`
+        places it at a caller-provided device address later."""
         self = cls.__new__(cls)
-        self._init(list(problems), dtype, tile=tile, ctx=ctx, lowered=lowered)
+        self._init(list(problems), dtype, tile=tile, ctx=ctx, lowered=lowered, defer=defer)
         return self
 
-    def _init(self, problems, dtype, *, tile, ctx, lowered):
+    def _init(self, problems, dtype, *, tile, ctx, lowered, defer=False):
+        self._defer = defer
         self.dt = np.dtype(dtype)
         if self.dt not in (np.dtype(np.float32), np.dtype(np.float64)):
             raise UnsupportedModelError(f"dtype {self.dt} is not supported")
         self.models = [m for m, _ in problems]
-        self.ctx = ctx if ctx is not None else nat.Context()
+        self.ctx = ctx if ctx is not None else (None if defer else nat.Context())
         t0 = time.perf_counter()
         low = lowered if lowered is not None else [lower_problem(m, p, self.dt) for m, p in problems]
         self.problems = [tabs for tabs, _ in low]
@@ -283,19 +286,30 @@ class GsProgram:
             arr["desc"][n:] = -1
             self.task_off.append(ar.add(arr, dedupe=False))
             self.n_tasks.append(npad)
+        self.tasks_host = [np.asarray(tl, dtype=np.int32).reshape(-1, 3) for tl in tasks]
         self.wdep_by_step = wdep
         code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
         L = nat.load_library()
         self.lds_by_step = [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
         self.total_bytes = _align(ar.size)
-        self.buf = nat.DeviceBuffer(self.total_bytes)
-        self.base = self.buf.ptr
-        D = self._desc_records(descs)
-        ar.parts.append((self.desc_off, D.tobytes()))
+        self._arena, self._descs = ar, descs
+        self.out_host = np.zeros(self.out_end, dtype=np.uint8)
+        self.buf = None
+        if not self._defer:
+            self.buf = nat.DeviceBuffer(self.total_bytes)
+            self.materialize(self.buf.ptr)
+
+    def materialize(self, ptr: int, upload_ctx=None):
+        """Write the descriptors for device address `ptr` and upload the image
+        there (a synchronous copy on `upload_ctx`, default the program's stream)."""
+        self.base = int(ptr)
+        self.D = self._desc_records(self._descs)
+        ar = self._arena
+        ar.parts.append((self.desc_off, self.D.tobytes()))
         image = ar.image()
         self.host_image_bytes = image.nbytes
-        nat.h2d(self.ctx, self.base, image)
-        self.out_host = np.zeros(self.out_end, dtype=np.uint8)
+        nat.h2d(upload_ctx or self.ctx, self.base, image)
+        ar.parts.pop()
 
     def _describe(self, model, p, t, r, pt, rg, ar, zero_off):
         dt = self.dt

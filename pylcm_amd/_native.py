@@ -164,6 +164,8 @@ _SIGS = {
     "lcm_event_destroy": ([vp], C.c_int),
     "lcm_event_record": ([vp, vp], C.c_int),
     "lcm_event_elapsed": ([vp, vp, C.POINTER(C.c_float)], C.c_int),
+    "lcm_event_query": ([vp], C.c_int),
+    "lcm_event_synchronize": ([vp], C.c_int),
     "lcm_copy_launch": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_abi_sizes": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
     "lcm_gs_lds_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),
@@ -326,6 +328,15 @@ class Event:
         h = vp()
         _check(lib().lcm_event_create(C.byref(h)), "lcm_event_create")
         self.handle = h
+
+    def query(self) -> bool:
+        st = lib().lcm_event_query(self.handle)
+        if st < 0:
+            _check(st, "lcm_event_query")
+        return st == 1
+
+    def synchronize(self):
+        call("lcm_event_synchronize", self.handle)
 
     def record(self, ctx: Context):
         call("lcm_event_record", ctx.handle, self.handle)

@@ -202,6 +202,20 @@ int lcm_event_record(lcm_ctx* ctx, void* ev) {
   return LCM_OK;
 }
 
+int lcm_event_query(void* ev) {
+  if (!ev) return fail(LCM_E_ARG, "lcm_event_query: null event");
+  const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(ev));
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return check_hip(e, "hipEventQuery");
+}
+
+int lcm_event_synchronize(void* ev) {
+  if (!ev) return fail(LCM_E_ARG, "lcm_event_synchronize: null event");
+  LCM_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev)));
+  return LCM_OK;
+}
+
 int lcm_event_elapsed(void* ev_start, void* ev_stop, float* ms) {
   if (!ev_start || !ev_stop || !ms) return fail(LCM_E_ARG, "lcm_event_elapsed: null argument");
   LCM_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev_stop)));

@@ -1,0 +1,343 @@
+"""Continuous-batching solve server for estimation-style request streams.
+
+Requests (model, params) arrive at any time. Each is lowered on the host by a
+worker thread into its own device image (tables, descriptors, V outputs); the
+serving loop admits it when the image fits in the server's share of free HBM,
+places it in one preallocated device slab and copies its descriptors into a
+shared descriptor pool. Every iteration then issues ONE fused grid-search
+launch holding the next backward step of every admitted request, whatever its
+model, grid size or horizon, so requests that arrive while others run join at
+the next iteration, and a request that finishes frees its memory for the
+queue. Value functions stay in device memory; `fetch` chooses what is copied
+back ("none", "first" = period 0, or "all").
+
+The reference solves one model at a time from a host loop
+(`src/_lcm/solution/backward_induction.py:155-275`); it has no server. The
+result of a request does not depend on what else is in flight: the same
+kernel, descriptors and tile produce it alone or in the mix (tests check
+bit equality).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from pylcm_amd import _native as nat
+from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_problem
+
+ALIGN = 256
+
+
+class _Slab:
+    """First-fit allocator over one device buffer (offsets in bytes, every
+    offset and length a multiple of `unit`)."""
+
+    def __init__(self, nbytes: int, unit: int = ALIGN):
+        self.unit = int(unit)
+        self.size = int(nbytes) // self.unit * self.unit
+        self.free = [(0, self.size)]  # sorted (offset, length)
+        self.used = 0
+
+    def alloc(self, nbytes: int):
+        n = max(1, (int(nbytes) + self.unit - 1) // self.unit) * self.unit
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, ln - n)
+                self.used += n
+                return off, n
+        return None
+
+    def release(self, off: int, n: int):
+        self.used -= n
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((o, ln))
+        self.free = merged
+
+
+@dataclass
+class Request:
+    rid: int
+    model: object
+    params: dict
+    t_submit: float
+    prog: GsProgram | None = None
+    error: BaseException | None = None
+    t_lowered: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    step: int = 0
+    slab: tuple | None = None
+    desc: tuple | None = None
+    done_event: object = None
+    result: dict | None = None
+    done: threading.Event = field(default_factory=threading.Event)
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_submit
+
+
+class SolveServer:
+    """Continuous batching of grid-search solves on one GPU.
+
+    `budget_bytes` caps the device memory the server uses (default: the given
+    fraction of the free HBM at start). `tile` is shared by all requests (the
+    fused launch runs one tile shape). `max_inflight` caps admitted requests.
+    """
+
+    def __init__(self, *, dtype=np.float32, tile: int = 128, budget_bytes: int | None = None,
+                 free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
+                 max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1):
+        self.dt = np.dtype(dtype)
+        self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
+        self.tile = int(tile)
+        self.fetch_mode = fetch
+        self.ctx = nat.Context()        # kernels and task uploads
+        self.copy_ctx = nat.Context()   # request images in, results out
+        _, free, _ = nat.device_info()
+        budget = int(free * free_fraction)
+        if budget_bytes is not None:
+            budget = min(budget, int(budget_bytes))
+        self.slab = _Slab(budget)
+        self.slab_buf = nat.DeviceBuffer(self.slab.size)
+        # descriptor slots in whole records: a task addresses its descriptor by index
+        self.desc_slab = _Slab(max_desc * DESC_DTYPE.itemsize, unit=DESC_DTYPE.itemsize)
+        self.desc_buf = nat.DeviceBuffer(self.desc_slab.size)
+        self.max_tasks = int(max_tasks)
+        self.max_inflight = int(max_inflight)
+        self.n_ring = 4
+        self.task_host = [nat.PinnedBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
+        self.task_dev = [nat.DeviceBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
+        self.ring_events = [nat.Event() for _ in range(self.n_ring)]
+        self.ring_used = [False] * self.n_ring
+        self.ring_i = 0
+        self._lowq: queue.Queue = queue.Queue()
+        self._ready: list[Request] = []      # lowered, waiting for memory
+        self._ready_lock = threading.Lock()
+        self.inflight: list[Request] = []
+        self.finishing: list[Request] = []
+        self.completed: list[Request] = []
+        self._next_id = 0
+        self.launches = 0
+        self.launch_tasks = 0
+        self._stop = False
+        self._workers = [threading.Thread(target=self._lower_worker, daemon=True) for _ in range(lower_threads)]
+        for w in self._workers:
+            w.start()
+
+    # -- client side --------------------------------------------------------------
+    def submit(self, model, params, t_submit: float | None = None) -> Request:
+        req = Request(self._next_id, model, params, t_submit if t_submit is not None else time.perf_counter())
+        self._next_id += 1
+        self._lowq.put(req)
+        return req
+
+    def close(self):
+        self._stop = True
+        for _ in self._workers:
+            self._lowq.put(None)
+
+    # -- host lowering ------------------------------------------------------------
+    def _lower_worker(self):
+        while True:
+            req = self._lowq.get()
+            if req is None:
+                return
+            try:
+                low = lower_problem(req.model, req.params, self.dt)
+                req.prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
+                                        ctx=self.ctx, defer=True)
+            except BaseException as e:  # surfaced to the client
+                req.error = e
+            req.t_lowered = time.perf_counter()
+            with self._ready_lock:
+                self._ready.append(req)
+
+    # -- serving loop ---------------------------------------------------------------
+    def _admit(self):
+        with self._ready_lock:
+            ready, self._ready = self._ready, []
+        keep = []
+        for req in ready:
+            if req.error is not None:
+                req.t_done = time.perf_counter()
+                req.done.set()
+                self.completed.append(req)
+                continue
+            if len(self.inflight) >= self.max_inflight:
+                keep.append(req)
+                continue
+            prog = req.prog
+            sl = self.slab.alloc(prog.total_bytes)
+            if sl is None:
+                if not self.inflight and not self.finishing and prog.total_bytes > self.slab.size:
+                    req.error = MemoryError(
+                        f"request needs {prog.total_bytes} B, more than the server's {self.slab.size} B")
+                    req.t_done = time.perf_counter()
+                    req.done.set()
+                    self.completed.append(req)
+                    continue
+                keep.append(req)
+                continue
+            ds = self.desc_slab.alloc(prog.n_desc * DESC_DTYPE.itemsize)
+            if ds is None:
+                self.slab.release(*sl)
+                keep.append(req)
+                continue
+            prog.materialize(self.slab_buf.ptr + sl[0], upload_ctx=self.copy_ctx)
+            nat.h2d(self.copy_ctx, self.desc_buf.ptr + ds[0], prog.D)
+            req.slab, req.desc = sl, ds
+            assert ds[0] % DESC_DTYPE.itemsize == 0 and ds[1] >= prog.n_desc * DESC_DTYPE.itemsize
+            req.desc_base = ds[0] // DESC_DTYPE.itemsize
+            req.t_admit = time.perf_counter()
+            req.step = 0
+            self.inflight.append(req)
+        if keep:
+            with self._ready_lock:
+                self._ready = keep + self._ready
+
+    def _launch(self):
+        """One fused launch: the next non-empty backward step of every request."""
+        parts, lds, variant = [], 0, 0
+        for req in self.inflight:
+            prog = req.prog
+            while req.step < prog.n_steps and len(prog.tasks_host[req.step]) == 0:
+                req.step += 1
+            if req.step >= prog.n_steps:
+                continue
+            t = prog.tasks_host[req.step].copy()
+            t[:, 0] += req.desc_base
+            parts.append(t)
+            lds = max(lds, prog.lds_by_step[req.step])
+            variant = max(variant, prog.wdep_by_step[req.step])
+            req.step += 1
+        if not parts:
+            return False
+        tasks = np.concatenate(parts)
+        n = len(tasks)
+        npad = (n + 7) // 8 * 8
+        if npad > self.max_tasks:
+            raise RuntimeError(f"{npad} tasks exceed the server's task buffer ({self.max_tasks})")
+        k = self.ring_i
+        self.ring_i = (k + 1) % self.n_ring
+        if self.ring_used[k]:
+            self.ring_events[k].synchronize()  # the copy from this host slot has finished
+        rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=npad)
+        rec["desc"][:n] = tasks[:, 0]
+        rec["sd"][:n] = tasks[:, 1]
+        rec["w0"][:n] = tasks[:, 2]
+        rec["pad"][:n] = 0
+        rec["desc"][n:] = -1
+        nat.call("lcm_memcpy_h2d_async", self.ctx.handle, C.c_void_p(self.task_dev[k].ptr),
+                 C.c_void_p(self.task_host[k].ptr), C.c_size_t(npad * TASK_DTYPE.itemsize))
+        nat.call("lcm_gs_launch_ex", self.ctx.handle, self.code, C.c_void_p(self.desc_buf.ptr),
+                 C.c_void_p(self.task_dev[k].ptr), npad, self.tile, C.c_size_t(lds), int(variant))
+        self.ring_events[k].record(self.ctx)
+        self.ring_used[k] = True
+        self.launches += 1
+        self.launch_tasks += n
+        still = []
+        for req in self.inflight:
+            if req.step >= req.prog.n_steps:
+                ev = nat.Event()
+                ev.record(self.ctx)
+                req.done_event = ev
+                self.finishing.append(req)
+            else:
+                still.append(req)
+        self.inflight = still
+        return True
+
+    def _retire(self, block: bool = False):
+        still = []
+        for req in self.finishing:
+            if block:
+                req.done_event.synchronize()
+            elif not req.done_event.query():
+                still.append(req)
+                continue
+            req.t_done = time.perf_counter()
+            self._collect(req)
+            self.slab.release(*req.slab)
+            self.desc_slab.release(*req.desc)
+            req.prog = None if self.fetch_mode != "keep" else req.prog
+            req.done.set()
+            self.completed.append(req)
+        self.finishing = still
+
+    def _collect(self, req: Request):
+        prog = req.prog
+        if self.fetch_mode == "none":
+            return
+        outs = prog.outs[0]
+        keys = sorted(outs) if self.fetch_mode in ("all", "keep") else [k for k in outs if k[0] == 0]
+        res = {}
+        for (t, r) in keys:
+            o = outs[(t, r)]
+            n = int(np.prod(o.shape, dtype=np.int64)) if o.shape else 1
+            v = np.empty(n, dtype=self.dt)
+            nat.d2h(self.copy_ctx, prog.base + o.v_off, v)
+            res.setdefault(t, {})[r] = v.reshape(o.shape)
+        req.result = res
+
+    def pending(self) -> int:
+        with self._ready_lock:
+            nready = len(self._ready)
+        return self._lowq.qsize() + nready + len(self.inflight) + len(self.finishing)
+
+    def step(self) -> bool:
+        """Admit, launch one fused step, retire finished requests. Returns
+        whether any work was issued."""
+        self._admit()
+        issued = self._launch()
+        self._retire()
+        return issued
+
+    def drain(self, timeout: float = 600.0):
+        t_end = time.perf_counter() + timeout
+        while self.pending() and time.perf_counter() < t_end:
+            if not self.step():
+                if self.finishing:
+                    self._retire(block=True)
+                else:
+                    time.sleep(0.0005)
+        self._retire(block=True)
+
+
+def serve_poisson(server: SolveServer, make_request, rate: float, duration: float, seed: int = 0):
+    """Submit requests with exponential inter-arrival times (rate per second)
+    for `duration` seconds while running the serving loop; then drain.
+    Returns the completed requests."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    next_t = t0 + rng.exponential(1.0 / rate)
+    submitted = []
+    while True:
+        now = time.perf_counter()
+        while next_t <= now and next_t - t0 < duration:
+            model, params = make_request(len(submitted))
+            submitted.append(server.submit(model, params, t_submit=next_t))
+            next_t += rng.exponential(1.0 / rate)
+        if next_t - t0 >= duration and now - t0 >= duration:
+            break
+        if not server.step():
+            if server.finishing:
+                server._retire()
+            time.sleep(0.0002)
+    server.drain()
+    return submitted

@@ -1,0 +1,71 @@
+"""Host-side pieces of the solve server (CPU): the device-slab allocator."""
+
+from pylcm_amd.serving import _Slab
+
+
+def test_slab_first_fit_and_coalescing():
+    s = _Slab(10 * 256)
+    a = s.alloc(300)      # 512 bytes
+    b = s.alloc(256)
+    c = s.alloc(1000)     # 1024
+    assert a == (0, 512) and b == (512, 256) and c == (768, 1024)
+    assert s.alloc(2000) is None
+    s.release(*b)
+    assert s.alloc(256) == (512, 256)
+    s.release(512, 256)
+    s.release(*a)
+    s.release(*c)
+    assert s.free == [(0, 2560)] and s.used == 0
+    assert s.alloc(2560) == (0, 2560)
+
+
+def test_descriptor_slots_are_whole_records():
+    """Descriptor slots must start at multiples of the record size: a task
+    names its descriptor by index into the pool."""
+    from pylcm_amd._gs_program import DESC_DTYPE
+
+    rec = DESC_DTYPE.itemsize
+    s = _Slab(100 * rec, unit=rec)
+    offs = [s.alloc(k * rec)[0] for k in (3, 7, 1, 5)]
+    assert all(o % rec == 0 for o in offs)
+    assert offs == [0, 3 * rec, 10 * rec, 11 * rec]
+
+
+def test_fused_tasks_address_each_requests_own_descriptors():
+    """Byte-level rehearsal of the descriptor pool: records copied at the slot
+    offsets the server allocates, read back by the index every task carries,
+    point into the request's own device image and cover the task's state."""
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import DESC_DTYPE, GsProgram
+
+    rng = np.random.default_rng(5)
+    reqs = []
+    for _ in range(12):
+        T, g = int(rng.integers(4, 9)), int(rng.integers(30, 300))
+        stoch = bool(rng.integers(0, 2))
+        reqs.append((ex.retirement_model(T, n_wealth=g, n_consumption=g, income_nodes=3 if stoch else None),
+                     ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0} if stoch else None)))
+    rec = DESC_DTYPE.itemsize
+    slab, dslab = _Slab(1 << 30), _Slab(4096 * rec, unit=rec)
+    pool = np.zeros(4096 * rec, dtype=np.uint8)
+    fake = 1 << 40
+    placed = []
+    for m, p in reqs:
+        prog = GsProgram.of([(m, p)], np.float32, tile=128, defer=True)
+        sl, ds = slab.alloc(prog.total_bytes), dslab.alloc(prog.n_desc * rec)
+        prog.base = fake + sl[0]
+        raw = np.frombuffer(prog._desc_records(prog._descs).tobytes(), dtype=np.uint8)
+        pool[ds[0]: ds[0] + raw.size] = raw
+        placed.append((prog, sl, ds[0] // rec))
+    table = np.frombuffer(pool.tobytes(), dtype=DESC_DTYPE)
+    for prog, sl, db in placed:
+        lo, hi = fake + sl[0], fake + sl[0] + sl[1]
+        for tasks in prog.tasks_host:
+            if len(tasks) == 0:
+                continue
+            r = table[tasks[:, 0] + db]
+            assert np.all((r["V_out"] >= lo) & (r["V_out"] < hi))
+            assert np.all((r["U"] >= lo) & (r["U"] < hi))
+            assert np.all(tasks[:, 1] < r["n_sd"]) and np.all(tasks[:, 2] < r["n_w"])
