@@ -61,6 +61,159 @@ def committed_traffic(key: str):
         return None
 
 
+def dcegm_leg(args):
+    """DC-EGM retirement model with FUES: T=50, 2000 exponentially spaced
+    savings nodes on [0, 500], 2000 wealth nodes, taste shocks 0.05, log-normal
+    income on 5 Gauss-Hermite nodes, CRRA 2, beta 0.96, r 0.03; float32 V with
+    FP64 candidate rows. Compared with grid search on a 4000-point consumption
+    grid, and with the FP64 oracle on a bounded sample (the same grids, T=3)."""
+    from pylcm_amd import _native as nat
+    from pylcm_amd import examples as ex
+    from pylcm_amd._egm_program import EgmProgram
+    from pylcm_amd._gs_program import GsProgram
+
+    dt = np.float32
+    T = args.dcegm_periods
+    m = ex.dcegm_retirement_model(T)
+    p = ex.dcegm_params(T)
+    t0 = time.perf_counter()
+    prog = EgmProgram(m, p, dt)
+    build_s = time.perf_counter() - t0
+    prog.run()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prog.run(sync=False)
+    prog.ctx.sync()
+    ms_solve = 1e3 * (time.perf_counter() - t0) / reps
+    e0, e1 = nat.Event(), nat.Event()
+    e0.record(prog.ctx)
+    prog.run(sync=False)
+    e1.record(prog.ctx)
+    kernel_ms = e0.elapsed_ms(e1)
+    V = prog.value_functions()
+    n_cols = sum(v[1] for v in prog._period.values())
+    log(f"dcegm: {ms_solve:.2f} ms per solve ({kernel_ms:.2f} ms on the stream), {len(prog._period)} periods, "
+        f"{n_cols} columns")
+    # the grid-search answer on a 4000-point consumption grid
+    g = GsProgram(ex.dcegm_retirement_model(T, grid_search=True), [p], dt)
+    g.run(use_graph=False)
+    g.fetch()
+    res = g.results(0)
+    diffs = []
+    for t in V:
+        for r in ("working_life", "retirement"):
+            if r in V[t] and r in res.get(t, {}):
+                a = np.asarray(V[t][r], np.float64)[..., 2:]
+                b = np.asarray(res[t][r][0], np.float64)[..., 2:]
+                diffs.append(np.abs(a - b) / np.abs(b))
+    d = np.concatenate([x.ravel() for x in diffs])
+    out = {
+        "workload": "DC-EGM retirement model, T=50, 2000 savings x 2000 wealth nodes, 5 GH income nodes, "
+                    "taste shocks 0.05, FUES on every column",
+        "ms_per_solve": round(ms_solve, 3),
+        "solves_per_s": round(1e3 / ms_solve, 2),
+        "stream_ms_per_solve": round(kernel_ms, 3),
+        "columns": n_cols,
+        "host_build_s": round(build_s, 3),
+        "vs_grid_search_4000": {"median_rel_diff": float(np.median(d)), "p99_rel_diff": float(np.percentile(d, 99)),
+                                "note": "grid resolution; the two lowest wealth nodes excluded"},
+    }
+    if not args.no_cpu:
+        from oracle import dcegm as od
+        from tests.parity import rel_err
+
+        Ts = 3
+        ms_ = ex.dcegm_retirement_model(Ts)
+        ps = ex.dcegm_params(Ts)
+        small = EgmProgram(ms_, ps, dt)
+        small.run()
+        Vg = small.value_functions()
+        t0 = time.perf_counter()
+        Vo, Co = od.solve(ms_, ps, grid_dtype=dt)
+        cpu_s = time.perf_counter() - t0
+        worst_v, worst_rows, same_kept = 0.0, 0.0, True
+        for t in Vo:
+            for r in Vo[t]:
+                worst_v = max(worst_v, rel_err(Vg[t][r], Vo[t][r]))
+                if r in Co[t]:
+                    cols = small.columns(t, r)
+                    for c, oc in enumerate(Co[t][r]):
+                        same_kept &= int(cols["counts"][c, 0]) == oc["n_kept"]
+                        for k in ("grid", "policy", "value"):
+                            worst_rows = max(worst_rows, rel_err(cols[k][c], oc[k]))
+        per_period = cpu_s / (Ts - 1)
+        out["parity_T3_full_grids"] = {"max_rel_V_err_vs_fp64": worst_v, "max_rel_carry_row_err": worst_rows,
+                                       "same_kept_counts": bool(same_kept)}
+        out["cpu_baseline"] = {"value": round(1.0 / (per_period * (T - 1)), 6), "unit": "model-solves/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"NumPy FP64 restatement (oracle/dcegm.py) of the same grids with T={Ts} "
+                                         f"({Ts - 1} periods, {cpu_s:.1f}s), scaled to {T - 1} periods"}
+        log(f"dcegm parity (T={Ts}): V {worst_v:.2e}, carry rows {worst_rows:.2e}, kept counts equal {same_kept}; "
+            f"cpu {cpu_s:.1f}s")
+    return out
+
+
+def serving_leg(args):
+    """Poisson arrivals (rate per second) for a fixed window into the
+    continuous-batching server; requests mix the deterministic and the AR(1)
+    income retirement models, horizons U{20..60}, grids U{200..4000}. The
+    same request sequence solved one at a time gives the plain baseline."""
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import GsProgram
+    from pylcm_amd.serving import SolveServer, serve_poisson
+
+    dt = np.float32
+    server = SolveServer(dtype=dt, tile=args.serve_tile, fetch="first", lower_threads=2)
+    make = ex.serving_request_factory(seed=7)
+    t0 = time.perf_counter()
+    reqs = serve_poisson(server, make, rate=args.serve_rate, duration=args.serve_seconds, seed=11)
+    wall = time.perf_counter() - t0
+    server.close()
+    ok = [r for r in reqs if r.error is None]
+    lat = np.array([r.latency for r in ok])
+    span = max(r.t_done for r in ok) - min(r.t_submit for r in ok)
+    log(f"serving: {len(ok)}/{len(reqs)} requests in {wall:.1f}s, {len(ok)/span:.1f} solves/s, "
+        f"p50 {1e3*np.percentile(lat, 50):.1f} ms, p99 {1e3*np.percentile(lat, 99):.1f} ms, "
+        f"{server.launches} fused launches")
+    # plain: the same sequence, one request at a time (lower, place, solve, fetch period 0)
+    make2 = ex.serving_request_factory(seed=7)
+    n_plain = min(args.serve_plain_sample, len(reqs))
+    plain = []
+    for i in range(n_plain):
+        m, p = make2(i)
+        t_a = time.perf_counter()
+        pr = GsProgram(m, [p], dt)
+        pr.run(use_graph=False)
+        o = pr.outs[0]
+        for (t, r), oo in o.items():
+            if t == 0:
+                n = int(np.prod(oo.shape, dtype=np.int64)) if oo.shape else 1
+                v = np.empty(n, dtype=dt)
+                from pylcm_amd import _native as nat
+                nat.d2h(pr.ctx, pr.base + oo.v_off, v)
+        plain.append(time.perf_counter() - t_a)
+        del pr
+    plain = np.array(plain)
+    return {
+        "workload": f"Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g}s; deterministic and AR(1)-income "
+                    "retirement models, T U{20..60}, grids U{200..4000}, float32, period-0 V returned",
+        "requests": len(reqs), "completed": len(ok),
+        "solves_per_s": round(len(ok) / span, 2),
+        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+        "fused_launches": server.launches,
+        "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
+        "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
+        "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
+        "plain_serial": {"solves_per_s": round(1.0 / plain.mean(), 2),
+                         "p50_ms": round(1e3 * float(np.percentile(plain, 50)), 2),
+                         "p99_ms": round(1e3 * float(np.percentile(plain, 99)), 2),
+                         "sample": n_plain},
+        "serving_vs_plain": round((len(ok) / span) * plain.mean(), 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +226,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--copy-calib-mib", type=int, default=256)
+    ap.add_argument("--no-dcegm", action="store_true")
+    ap.add_argument("--dcegm-periods", type=int, default=50)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-plain-sample", type=int, default=40)
+    ap.add_argument("--serve-tile", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -217,6 +376,9 @@ def main():
                   "checked_draws": len(order)}
         log(f"cpu baseline: {cpu['value']:.3f} solves/s on {threads} threads; parity {parity}")
 
+    dcegm = None if args.no_dcegm else dcegm_leg(args)
+    serving = serving_leg(args) if args.serve_seconds > 0 else None
+
     key = f"draws={args.draws} periods={T} tile={prog.tile} dtype=f32"
     traffic = committed_traffic(key)
     out = {
@@ -266,6 +428,8 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "host_lowering_s": round(lower_s, 2),
+        "dcegm": dcegm,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

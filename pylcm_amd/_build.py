@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import hashlib
 from dataclasses import dataclass, field
+from functools import cached_property
 from typing import Any
 
 import numpy as np
@@ -64,31 +65,32 @@ class RegimeLayout:
     c_name: str | None
     c_grid: Any
 
-    @property
+    # sizes are read in every per-period lowering step, so they are computed once
+    @cached_property
     def sd_sizes(self):
         return tuple(g.n_points for g in self.sd_grids)
 
-    @property
+    @cached_property
     def da_sizes(self):
         return tuple(g.n_points for g in self.da_grids)
 
-    @property
+    @cached_property
     def n_sd(self):
         return int(np.prod(self.sd_sizes, dtype=np.int64)) if self.sd_sizes else 1
 
-    @property
+    @cached_property
     def n_w(self):
         return self.w_grid.n_points if self.w_grid is not None else 1
 
-    @property
+    @cached_property
     def n_d(self):
         return int(np.prod(self.da_sizes, dtype=np.int64)) if self.da_sizes else 1
 
-    @property
+    @cached_property
     def n_c(self):
         return self.c_grid.n_points if self.c_grid is not None else 1
 
-    @property
+    @cached_property
     def V_shape(self):
         return self.sd_sizes + ((self.n_w,) if self.w_grid is not None else ())
 
@@ -440,13 +442,67 @@ class PeriodLowering:
         ids = ax.to4(np.asarray(out, dtype=np.float64))[:, 0, :, 0]
         return ids.astype(np.int64)
 
+    def _ids_all_periods(self, lay, tr):
+        """A deterministic regime transition evaluated for every period at once
+        (age and period on a leading axis), or None when the function does not
+        broadcast over them (then it is evaluated period by period)."""
+        key = ("ids_all", lay.name, id(tr))
+        if key in self._memo:
+            return self._memo[key][0]
+        ids = None
+        if not (lay.c_name and lay.c_name in _arg_closure(lay, tr)):
+            try:
+                ax = _Axes(lay, self.rg, full_w=False)
+                T = self.model.n_periods
+                lead = (T,) + (1,) * ax.ndim
+                variables = {k: v for k, v in ax.vars.items() if k != lay.c_name}
+                variables["age"] = np.asarray([self.model.ages.period_to_age(t) for t in range(T)]).reshape(lead)
+                variables["period"] = np.arange(T).reshape(lead)
+                ev = Evaluator(regime_name=lay.name, functions=_functions_of(lay), variables=variables,
+                               params=self.params, fixed_params=self.fixed, param_dtype=self.rg.dtype)
+                out = np.asarray(ev.call("next_regime", tr), dtype=np.float64)
+                full = lay.sd_sizes + (1,) + lay.da_sizes + (1,)
+                out = np.broadcast_to(out, (T,) + full)
+                ids = out.reshape(T, lay.n_sd, 1, lay.n_d, 1)[:, :, 0, :, 0].astype(np.int64)
+            except Exception:  # not vectorisable over the age: per-period path
+                ids = None
+        self._memo[key] = (ids, tr)
+        return ids
+
+    def period_key(self, lay, period):
+        """A key under which the tables of (lay, period) equal those of any
+        other period with the same key, or None. Holds when every function the
+        lowering evaluates is age-free except a deterministic regime transition
+        that broadcasts over the age (its ids then enter the key)."""
+        reg = lay.regime
+        T = self.model.n_periods
+        funcs = [reg.functions["utility"]] + [f for f in reg.constraints.values() if f is not None]
+        if lay.terminal:
+            return ("terminal", lay.name) if self._age_free(lay, funcs) else None
+        if period == T - 1:
+            return None
+        tr = reg.transition
+        if isinstance(tr, MarkovTransition) or hasattr(tr, "items"):
+            return None
+        for f in reg.state_transitions.values():
+            funcs += list(f.values()) if hasattr(f, "items") else [f]
+        if not self._age_free(lay, funcs):
+            return None
+        ids_all = self._ids_all_periods(lay, tr)
+        if ids_all is None:
+            return None
+        age_next = self.model.ages.period_to_age(period + 1)
+        active_next = tuple(r for r in self.model.regime_names if self.model.layouts[r].regime.active(age_next))
+        return ("period", lay.name, active_next, ids_all[period].tobytes())
+
     def regime_probs(self, lay, period):
         """p[sd, d, regime_code] of moving to each regime."""
         n_reg = len(self.model.regime_names)
         p = np.zeros((lay.n_sd, lay.n_d, n_reg))
         tr = lay.regime.transition
         if not isinstance(tr, MarkovTransition) and not hasattr(tr, "items"):
-            ids = self._vectorized_ids(lay, period, tr)
+            ids_all = self._ids_all_periods(lay, tr)
+            ids = ids_all[period] if ids_all is not None else self._vectorized_ids(lay, period, tr)
             if np.any((ids < 0) | (ids >= n_reg)):
                 raise InvalidParamsError(f"regime {lay.name!r}: next regime id out of range")
             np.put_along_axis(p, ids[..., None], 1.0, axis=-1)

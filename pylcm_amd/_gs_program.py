@@ -72,10 +72,12 @@ class _Arena:
         self.size = off + max(int(nbytes), 1)
         return off
 
-    def image(self) -> np.ndarray:
-        buf = np.zeros(_align(max(self.size, 1)), dtype=np.uint8)
+    def image(self, start: int = 0) -> np.ndarray:
+        """Bytes [start, size) of the arena (reserved space reads as zeros)."""
+        buf = np.zeros(_align(max(self.size, 1)) - start, dtype=np.uint8)
         for off, b in self.parts:
-            buf[off: off + len(b)] = np.frombuffer(b, dtype=np.uint8)
+            if off >= start:
+                buf[off - start: off - start + len(b)] = np.frombuffer(b, dtype=np.uint8)
         return buf
 
 
@@ -156,6 +158,7 @@ def lower_problem(model, params, dtype):
     rg = resolve_grids(model.layouts, params, model.fixed_params, dtype)
     low = PeriodLowering(model, params, rg)
     tabs = {}
+    same = {}  # period key -> tables shared by every period with that key
     for t in range(model.n_periods):
         age = model.ages.period_to_age(t)
         for r in model.regime_names:
@@ -164,7 +167,13 @@ def lower_problem(model, params, dtype):
                 continue
             if lay.is_dcegm:
                 raise UnsupportedModelError("DC-EGM regimes are solved by the DC-EGM program")
+            key = low.period_key(lay, t)
+            if key is not None and key in same:
+                tabs[(t, r)] = same[key]
+                continue
             tabs[(t, r)] = low.lower(lay, t)
+            if key is not None:
+                same[key] = tabs[(t, r)]
     return tabs, rg
 
 
@@ -224,7 +233,6 @@ class GsProgram:
         isz = dt.itemsize
         S = self.n_steps
         ar = _Arena()
-        zero_off = ar.add(np.zeros(1, dtype=dt))
         self.outs: list[dict] = []
         # outputs are reserved first so that descriptors can point at them
         for model, tabs in zip(self.models, self.problems):
@@ -238,6 +246,8 @@ class GsProgram:
                                  lay.V_shape + ((lay.n_d,) if taste else ()))
             self.outs.append(o)
         self.out_end = ar.size
+        self.out_start = _align(self.out_end)  # every table is placed from here on
+        zero_off = ar.add(np.zeros(1, dtype=dt))
         descs = []
         tasks = [[] for _ in range(S)]
         wdep = [0] * S  # kernel variant per step: the most general any descriptor needs
@@ -245,12 +255,13 @@ class GsProgram:
         kmax = [0] * S
         self.flops_by_step = [0.0] * S
         self.bytes_by_step = [0.0] * S
+        dcache = {}
         for p, (model, tabs, rg) in enumerate(zip(self.models, self.problems, self._rg)):
             order = sorted(tabs.items(), key=lambda kv: (-kv[0][0], model.regime_code(kv[0][1])))
             for (t, r), pt in order:
                 s = self.step_of(p, t)
                 lay = model.layouts[r]
-                d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
+                d = self._describe_cached(model, p, t, r, pt, rg, ar, zero_off, dcache)
                 di = len(descs)
                 descs.append(d)
                 wdep[s] = max(wdep[s], d["variant"])
@@ -306,10 +317,33 @@ class GsProgram:
         self.D = self._desc_records(self._descs)
         ar = self._arena
         ar.parts.append((self.desc_off, self.D.tobytes()))
-        image = ar.image()
+        # the V and policy outputs come first and are written by the kernels
+        # before any read, so only the tables behind them are uploaded
+        image = ar.image(self.out_start)
         self.host_image_bytes = image.nbytes
-        nat.h2d(upload_ctx or self.ctx, self.base, image)
+        nat.h2d(upload_ctx or self.ctx, self.base + self.out_start, image)
         ar.parts.pop()
+
+    def _describe_cached(self, model, p, t, r, pt, rg, ar, zero_off, cache):
+        """`_describe`, reusing the work of an earlier period whose tables are
+        the same objects (the lowering shares tables that do not depend on
+        the age): only the output and next-period V offsets differ."""
+        key = (id(model), p, r, id(pt.U), id(pt.F), id(pt.A), id(pt.B), pt.terminal, pt.beta,
+               pt.taste_scale, pt.next_grid, tuple((n, id(M)) for n, M in pt.targets),
+               tuple((n, id(m)) for n, m in pt.scalars))
+        hit = cache.get(key)
+        if hit is None:
+            d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
+            cache[key] = (d, pt)  # pt keeps the keyed tables alive
+            return d
+        d = dict(hit[0])
+        out = self.outs[p][(t, r)]
+        d["V_out"], d["pol_out"] = out.v_off, out.p_off
+        if not pt.terminal:
+            d["Vn"] = [self.outs[p][(t + 1, name)].v_off for name, _ in pt.targets]
+            d["Vs"] = [self.outs[p][(t + 1, name)].v_off if name is not None else zero_off
+                       for name, _ in pt.scalars]
+        return d
 
     def _describe(self, model, p, t, r, pt, rg, ar, zero_off):
         dt = self.dt
@@ -395,6 +429,29 @@ class GsProgram:
         lin = (not d["g_n"]) or d["g_kind"] == 0
         d["variant"] = 2 if wdep else (0 if (iv is not None and lin) else 1)
         return d
+
+    def relocatable_records(self):
+        """(records for base address 0, {field: mask of its pointer entries}):
+        `records_at(base)` turns them into the descriptors for any base."""
+        saved = getattr(self, "base", None)
+        self.base = 0
+        D0 = self._desc_records(self._descs)
+        self.base = 1
+        D1 = self._desc_records(self._descs)
+        self.base = saved
+        masks = {}
+        for f in D0.dtype.names:
+            m = D1[f] != D0[f]
+            if np.any(m):
+                masks[f] = m
+        return D0, masks
+
+    @staticmethod
+    def records_at(D0, masks, base: int):
+        D = D0.copy()
+        for f, m in masks.items():
+            D[f][m] += np.uint64(base)
+        return D
 
     def _desc_records(self, descs):
         """Descriptor records, filled column by column."""

@@ -516,3 +516,33 @@ def dcegm_params(n_periods, *, discount_factor=0.96, crra=2.0, disutility_of_wor
     if taste_scale is not None:
         p["working_life"]["taste_shocks"] = {"scale": taste_scale}
     return p
+
+
+# -- serving request mix ---------------------------------------------------------
+
+
+def serving_request_factory(seed: int = 0, horizons=(20, 60), grids=(200, 4000), income_nodes: int = 7):
+    """`make(i) -> (model, params)` drawing the serving workload's requests:
+    half deterministic (the T=10 workload's economics), half with the AR(1)
+    income shock, horizon U{20..60}, wealth = consumption grid U{200..4000},
+    and parameters drawn as in `draw_batch_params`. Models are shared between
+    requests of equal shape."""
+    rng = np.random.default_rng(seed)
+    models = {}
+
+    def make(i):
+        stoch = bool(rng.integers(0, 2))
+        T = int(rng.integers(horizons[0], horizons[1] + 1))
+        g = int(rng.integers(grids[0], grids[1] + 1))
+        key = (stoch, T, g)
+        if key not in models:
+            models[key] = retirement_model(T, n_wealth=g, n_consumption=g,
+                                           income_nodes=income_nodes if stoch else None)
+        income = None
+        if stoch:
+            income = {"rho": float(rng.uniform(0.8, 0.97)), "sigma": float(rng.uniform(0.05, 0.2)), "mu": 0.0}
+        p = retirement_params(T, discount_factor=float(rng.uniform(0.90, 0.99)),
+                              crra=float(rng.uniform(1.1, 3.0)), income=income)
+        return models[key], p
+
+    return make

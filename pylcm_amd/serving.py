@@ -1,10 +1,10 @@
 """Continuous-batching solve server for estimation-style request streams.
 
-Requests (model, params) arrive at any time. Each is lowered on the host by a
-worker thread into its own device image (tables, descriptors, V outputs); the
-serving loop admits it when the image fits in the server's share of free HBM,
-places it in one preallocated device slab and copies its descriptors into a
-shared descriptor pool. Every iteration then issues ONE fused grid-search
+Requests (model, params) arrive at any time. A worker thread lowers each into
+its own device image (tables, descriptors, V outputs), waits until the image
+fits in the server's share of free HBM, places it in one preallocated device
+slab (its own upload stream) and copies its descriptors into a shared
+descriptor pool. Every iteration of the serving loop issues ONE fused grid-search
 launch holding the next backward step of every admitted request, whatever its
 model, grid size or horizon, so requests that arrive while others run join at
 the next iteration, and a request that finishes frees its memory for the
@@ -83,6 +83,7 @@ class Request:
     step: int = 0
     slab: tuple | None = None
     desc: tuple | None = None
+    tasks: list | None = None
     done_event: object = None
     result: dict | None = None
     done: threading.Event = field(default_factory=threading.Event)
@@ -100,7 +101,7 @@ class SolveServer:
     fused launch runs one tile shape). `max_inflight` caps admitted requests.
     """
 
-    def __init__(self, *, dtype=np.float32, tile: int = 128, budget_bytes: int | None = None,
+    def __init__(self, *, dtype=np.float32, tile: int = 256, budget_bytes: int | None = None,
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1):
         self.dt = np.dtype(dtype)
@@ -120,21 +121,27 @@ class SolveServer:
         self.desc_buf = nat.DeviceBuffer(self.desc_slab.size)
         self.max_tasks = int(max_tasks)
         self.max_inflight = int(max_inflight)
-        self.n_ring = 4
+        # at most two fused launches queued on the stream: while one runs, the
+        # next one gathers every step that becomes ready, so launches stay large
+        self.n_ring = 2
         self.task_host = [nat.PinnedBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
         self.task_dev = [nat.DeviceBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
         self.ring_events = [nat.Event() for _ in range(self.n_ring)]
         self.ring_used = [False] * self.n_ring
         self.ring_i = 0
         self._lowq: queue.Queue = queue.Queue()
-        self._ready: list[Request] = []      # lowered, waiting for memory
+        self._ready: list[Request] = []      # placed on the device, not yet in a launch
         self._ready_lock = threading.Lock()
+        self._mem_cv = threading.Condition()  # guards the slabs; notified on retirement
+        self._n_placed = 0
         self.inflight: list[Request] = []
         self.finishing: list[Request] = []
         self.completed: list[Request] = []
         self._next_id = 0
         self.launches = 0
         self.launch_tasks = 0
+        self.timing = {"admit": 0.0, "launch": 0.0, "retire": 0.0}
+        self._launch_events = []  # (start, end) per fused launch, for the GPU busy time
         self._stop = False
         self._workers = [threading.Thread(target=self._lower_worker, daemon=True) for _ in range(lower_threads)]
         for w in self._workers:
@@ -151,103 +158,116 @@ class SolveServer:
         self._stop = True
         for _ in self._workers:
             self._lowq.put(None)
+        with self._mem_cv:
+            self._mem_cv.notify_all()
 
-    # -- host lowering ------------------------------------------------------------
+    # -- host lowering and admission (worker threads) ------------------------------
     def _lower_worker(self):
+        copy_ctx = nat.Context()  # this worker's upload stream
         while True:
             req = self._lowq.get()
             if req is None:
                 return
             try:
                 low = lower_problem(req.model, req.params, self.dt)
-                req.prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
-                                        ctx=self.ctx, defer=True)
+                prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
+                                    ctx=self.ctx, defer=True)
+                D0, masks = prog.relocatable_records()
+                image = prog._arena.image(prog.out_start)
+                req.prog = prog
+                req.t_lowered = time.perf_counter()
+                self._place(req, copy_ctx, image, D0, masks)
             except BaseException as e:  # surfaced to the client
                 req.error = e
-            req.t_lowered = time.perf_counter()
-            with self._ready_lock:
-                self._ready.append(req)
+                req.t_done = time.perf_counter()
+                req.done.set()
+                with self._ready_lock:
+                    self.completed.append(req)
+
+    def _place(self, req, copy_ctx, image, D0, masks):
+        """Wait for device memory, upload the tables and the descriptors, and
+        hand the request to the serving loop."""
+        prog = req.prog
+        need_desc = prog.n_desc * DESC_DTYPE.itemsize
+        with self._mem_cv:
+            while True:
+                if self._stop:
+                    raise RuntimeError("server closed")
+                sl = self.slab.alloc(prog.total_bytes) if self._n_placed < self.max_inflight else None
+                ds = self.desc_slab.alloc(need_desc) if sl is not None else None
+                if sl is not None and ds is not None:
+                    self._n_placed += 1
+                    break
+                if sl is not None:
+                    self.slab.release(*sl)
+                if prog.total_bytes > self.slab.size:
+                    raise MemoryError(
+                        f"request needs {prog.total_bytes} B, more than the server's {self.slab.size} B")
+                self._mem_cv.wait(timeout=0.02)
+        assert ds[0] % DESC_DTYPE.itemsize == 0 and ds[1] >= need_desc
+        base = self.slab_buf.ptr + sl[0]
+        prog.base = base
+        nat.h2d(copy_ctx, base + prog.out_start, image)
+        D = GsProgram.records_at(D0, masks, base)
+        nat.h2d(copy_ctx, self.desc_buf.ptr + ds[0], D)
+        desc_base = ds[0] // DESC_DTYPE.itemsize
+        steps = []
+        for t in prog.tasks_host:
+            rec = np.zeros(len(t), dtype=TASK_DTYPE)
+            if len(t):
+                rec["desc"] = t[:, 0] + desc_base
+                rec["sd"] = t[:, 1]
+                rec["w0"] = t[:, 2]
+            steps.append(rec)
+        req.tasks = steps
+        req.slab, req.desc = sl, ds
+        req.t_admit = time.perf_counter()
+        req.step = 0
+        with self._ready_lock:
+            self._ready.append(req)
 
     # -- serving loop ---------------------------------------------------------------
     def _admit(self):
         with self._ready_lock:
             ready, self._ready = self._ready, []
-        keep = []
-        for req in ready:
-            if req.error is not None:
-                req.t_done = time.perf_counter()
-                req.done.set()
-                self.completed.append(req)
-                continue
-            if len(self.inflight) >= self.max_inflight:
-                keep.append(req)
-                continue
-            prog = req.prog
-            sl = self.slab.alloc(prog.total_bytes)
-            if sl is None:
-                if not self.inflight and not self.finishing and prog.total_bytes > self.slab.size:
-                    req.error = MemoryError(
-                        f"request needs {prog.total_bytes} B, more than the server's {self.slab.size} B")
-                    req.t_done = time.perf_counter()
-                    req.done.set()
-                    self.completed.append(req)
-                    continue
-                keep.append(req)
-                continue
-            ds = self.desc_slab.alloc(prog.n_desc * DESC_DTYPE.itemsize)
-            if ds is None:
-                self.slab.release(*sl)
-                keep.append(req)
-                continue
-            prog.materialize(self.slab_buf.ptr + sl[0], upload_ctx=self.copy_ctx)
-            nat.h2d(self.copy_ctx, self.desc_buf.ptr + ds[0], prog.D)
-            req.slab, req.desc = sl, ds
-            assert ds[0] % DESC_DTYPE.itemsize == 0 and ds[1] >= prog.n_desc * DESC_DTYPE.itemsize
-            req.desc_base = ds[0] // DESC_DTYPE.itemsize
-            req.t_admit = time.perf_counter()
-            req.step = 0
-            self.inflight.append(req)
-        if keep:
-            with self._ready_lock:
-                self._ready = keep + self._ready
+        self.inflight.extend(ready)
 
     def _launch(self):
         """One fused launch: the next non-empty backward step of every request."""
-        parts, lds, variant = [], 0, 0
+        k = self.ring_i
+        if self.ring_used[k]:
+            self.ring_events[k].synchronize()  # the copy from this host slot has finished
+        rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
+        n, lds, variant = 0, 0, 0
         for req in self.inflight:
             prog = req.prog
-            while req.step < prog.n_steps and len(prog.tasks_host[req.step]) == 0:
+            while req.step < prog.n_steps and len(req.tasks[req.step]) == 0:
                 req.step += 1
             if req.step >= prog.n_steps:
                 continue
-            t = prog.tasks_host[req.step].copy()
-            t[:, 0] += req.desc_base
-            parts.append(t)
+            t = req.tasks[req.step]
+            if n + len(t) + 8 > self.max_tasks:
+                break  # the rest join the next launch
+            rec[n: n + len(t)] = t
+            n += len(t)
             lds = max(lds, prog.lds_by_step[req.step])
             variant = max(variant, prog.wdep_by_step[req.step])
             req.step += 1
-        if not parts:
+        if n == 0:
             return False
-        tasks = np.concatenate(parts)
-        n = len(tasks)
         npad = (n + 7) // 8 * 8
-        if npad > self.max_tasks:
-            raise RuntimeError(f"{npad} tasks exceed the server's task buffer ({self.max_tasks})")
-        k = self.ring_i
+        rec["desc"][n:npad] = -1
         self.ring_i = (k + 1) % self.n_ring
-        if self.ring_used[k]:
-            self.ring_events[k].synchronize()  # the copy from this host slot has finished
-        rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=npad)
-        rec["desc"][:n] = tasks[:, 0]
-        rec["sd"][:n] = tasks[:, 1]
-        rec["w0"][:n] = tasks[:, 2]
-        rec["pad"][:n] = 0
-        rec["desc"][n:] = -1
+        e_start = nat.Event()
+        e_start.record(self.ctx)
         nat.call("lcm_memcpy_h2d_async", self.ctx.handle, C.c_void_p(self.task_dev[k].ptr),
                  C.c_void_p(self.task_host[k].ptr), C.c_size_t(npad * TASK_DTYPE.itemsize))
         nat.call("lcm_gs_launch_ex", self.ctx.handle, self.code, C.c_void_p(self.desc_buf.ptr),
                  C.c_void_p(self.task_dev[k].ptr), npad, self.tile, C.c_size_t(lds), int(variant))
         self.ring_events[k].record(self.ctx)
+        e_end = nat.Event()
+        e_end.record(self.ctx)
+        self._launch_events.append((e_start, e_end))
         self.ring_used[k] = True
         self.launches += 1
         self.launch_tasks += n
@@ -264,7 +284,7 @@ class SolveServer:
         return True
 
     def _retire(self, block: bool = False):
-        still = []
+        still, freed = [], False
         for req in self.finishing:
             if block:
                 req.done_event.synchronize()
@@ -273,12 +293,20 @@ class SolveServer:
                 continue
             req.t_done = time.perf_counter()
             self._collect(req)
-            self.slab.release(*req.slab)
-            self.desc_slab.release(*req.desc)
+            with self._mem_cv:
+                self.slab.release(*req.slab)
+                self.desc_slab.release(*req.desc)
+                self._n_placed -= 1
+            freed = True
             req.prog = None if self.fetch_mode != "keep" else req.prog
+            req.tasks = None
             req.done.set()
-            self.completed.append(req)
+            with self._ready_lock:
+                self.completed.append(req)
         self.finishing = still
+        if freed:
+            with self._mem_cv:
+                self._mem_cv.notify_all()
 
     def _collect(self, req: Request):
         prog = req.prog
@@ -296,17 +324,29 @@ class SolveServer:
         req.result = res
 
     def pending(self) -> int:
+        """Requests submitted and not yet completed."""
         with self._ready_lock:
-            nready = len(self._ready)
-        return self._lowq.qsize() + nready + len(self.inflight) + len(self.finishing)
+            return self._next_id - len(self.completed)
 
     def step(self) -> bool:
         """Admit, launch one fused step, retire finished requests. Returns
         whether any work was issued."""
+        t0 = time.perf_counter()
         self._admit()
+        t1 = time.perf_counter()
         issued = self._launch()
+        t2 = time.perf_counter()
         self._retire()
+        t3 = time.perf_counter()
+        self.timing["admit"] += t1 - t0
+        self.timing["launch"] += t2 - t1
+        self.timing["retire"] += t3 - t2
         return issued
+
+    def gpu_busy_seconds(self) -> float:
+        """Sum of the fused launches' durations on the stream (task upload
+        included), from events; call after `drain`."""
+        return sum(a.elapsed_ms(b) for a, b in self._launch_events) / 1e3
 
     def drain(self, timeout: float = 600.0):
         t_end = time.perf_counter() + timeout
