@@ -34,9 +34,11 @@ __device__ __forceinline__ T zero_safe(T w, T v) {
   return (w == T(0)) ? T(0) : w * v;
 }
 
+// A class test: unlike `x - x == 0` it cannot be rewritten by fp contraction
+// (an fma of a product with its own negation returns the rounding error).
 template <typename T>
 __device__ __forceinline__ bool is_finite(T x) {
-  return x - x == T(0);
+  return __builtin_isfinite(x);
 }
 
 }  // namespace lcm

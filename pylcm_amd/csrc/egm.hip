@@ -41,7 +41,7 @@ __device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000
 __device__ __forceinline__ double pinf() { return __longlong_as_double(0x7ff0000000000000LL); }
 __device__ __forceinline__ double ninf() { return __longlong_as_double((long long)0xfff0000000000000ULL); }
 __device__ __forceinline__ bool isnan_(double x) { return x != x; }
-__device__ __forceinline__ bool isfin(double x) { return x - x == 0.0; }
+__device__ __forceinline__ bool isfin(double x) { return __builtin_isfinite(x); }
 __device__ __forceinline__ bool isneginf_(double x) { return x == ninf(); }
 
 // maximum that propagates NaN (jnp.maximum)

@@ -176,6 +176,12 @@ __device__ __forceinline__ void block_merge(Running<T>& r, T* red_v, int* red_i,
   }
 }
 
+// One entry of the folded EV row: value at a node and slope to the next.
+template <typename T>
+struct alignas(sizeof(T) * 2) Pair {
+  T v, dv;
+};
+
 // Everything the action loop reads (block-uniform except w / valid / wr).
 template <typename T>
 struct LoopArgs {
@@ -186,6 +192,7 @@ struct LoopArgs {
   const T* Ug;
   const T* Ag;
   const T* Bg;
+  const Pair<T>* pairs;  // folded EV row (fast variant)
   const LcmGsDesc* D;
   Grid1D<T> g;
   T w, beta, cs;
@@ -247,6 +254,93 @@ __device__ __forceinline__ void staged_loop(const LoopArgs<T>& L, Running<T>& ru
     offer(run, q0, f0, L.d * L.n_c + L.c0 + i);
     offer(run, q1, f1, L.d * L.n_c + L.c0 + i1);
   }
+}
+
+// Folded loop of the fast variant for the common case (interval feasibility,
+// linear next grid, finite EV row with at least two points). Staging folded
+// the grid into each action, so the coordinate of the next state is one fma,
+// coord = A' * w + B' with A' = A / step and B' = (B - start) / step, and the
+// EV row is held as pairs (s_k, s_{k+1} - s_k) of s = beta * (EV + cs), so
+// Q = U + s_lo + t * slope. The block runs this loop only on chunks whose
+// magnitudes (|coord| <= 2^22, |s|, |slope| <= 1e30, U not NaN) rule out
+// overflow and NaN; the other chunks take the staged loop. LO / HI drop the
+// bound checks a chunk does not need (all lower bounds -inf / all upper
+// bounds +inf). Four actions per iteration give the scheduler independent LDS
+// reads. The update is `offer`'s first-argmax rule, with NaN as the running
+// best while no action is feasible: the first feasible action passes
+// !(q <= best) whatever its value, later ones need q > best.
+template <typename T, int SPLIT, bool LO, bool HI, bool INTERP>
+__device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& run) {
+  const T w = L.w;
+  const T top = T(L.g_n - 2);
+  const Pair<T>* __restrict__ pr = L.pairs;
+  const int base = L.d * L.n_c + L.c0;
+  T best = run.idx < 0 ? T(NAN) : run.best;
+  int idx = run.idx;
+  auto feas_of = [&](const Act<T>& a, int i) -> bool {
+    bool f = L.valid;
+    if constexpr (LO) f = f & (w >= a.lo);
+    if constexpr (HI) f = f & (w <= L.his[i]);
+    return f;
+  };
+  auto q_of = [&](const Act<T>& a) -> T {
+    if constexpr (!INTERP) return a.U;  // staged as U + beta * cs
+    const T coord = a.A * w + a.B;
+    const T fl = clamp_index(T(floor(coord)), top);
+    const Pair<T> p = pr[static_cast<int>(fl)];
+    return a.U + (p.v + (coord - fl) * p.dv);
+  };
+  auto update = [&](T q, bool f, int c) {
+    const bool take = f & !(q <= best);
+    best = take ? q : best;
+    idx = take ? c : idx;
+  };
+  int i = L.grp;
+  for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
+    const Act<T> a0 = L.act[i];
+    const Act<T> a1 = L.act[i + SPLIT];
+    const Act<T> a2 = L.act[i + 2 * SPLIT];
+    const Act<T> a3 = L.act[i + 3 * SPLIT];
+    const bool f0 = feas_of(a0, i);
+    const bool f1 = feas_of(a1, i + SPLIT);
+    const bool f2 = feas_of(a2, i + 2 * SPLIT);
+    const bool f3 = feas_of(a3, i + 3 * SPLIT);
+    if (__builtin_amdgcn_ballot_w64(f0 | f1 | f2 | f3) == 0) continue;  // no lane of this wave takes any of the four
+    const T q0 = q_of(a0);
+    const T q1 = q_of(a1);
+    const T q2 = q_of(a2);
+    const T q3 = q_of(a3);
+    update(q0, f0, base + i);
+    update(q1, f1, base + i + SPLIT);
+    update(q2, f2, base + i + 2 * SPLIT);
+    update(q3, f3, base + i + 3 * SPLIT);
+  }
+  for (; i < L.cn; i += SPLIT) {
+    const Act<T> a = L.act[i];
+    const bool f = feas_of(a, i);
+    if (__builtin_amdgcn_ballot_w64(f) == 0) continue;
+    update(q_of(a), f, base + i);
+  }
+  run.best = idx < 0 ? T(-INFINITY) : best;
+  run.idx = idx;
+}
+
+template <typename T, int SPLIT, bool INTERP>
+__device__ __forceinline__ void folded_bounds(const LoopArgs<T>& L, bool lo, bool hi, Running<T>& run) {
+  if (lo) {
+    if (hi) folded_loop<T, SPLIT, true, true, INTERP>(L, run);
+    else folded_loop<T, SPLIT, true, false, INTERP>(L, run);
+  } else {
+    if (hi) folded_loop<T, SPLIT, false, true, INTERP>(L, run);
+    else folded_loop<T, SPLIT, false, false, INTERP>(L, run);
+  }
+}
+
+template <typename T, int SPLIT>
+__device__ __forceinline__ void folded_dispatch(const LoopArgs<T>& L, bool interp, bool lo, bool hi,
+                                                Running<T>& run) {
+  if (interp) folded_bounds<T, SPLIT, true>(L, lo, hi, run);
+  else folded_bounds<T, SPLIT, false>(L, lo, hi, run);
 }
 
 // The action loop over one chunk, specialised on the kernel variant, the
@@ -353,6 +447,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const T w = static_cast<const T*>(D->w_pts)[wr];
   const T w_first = static_cast<const T*>(D->w_pts)[tk.w0];
   const T w_last = static_cast<const T*>(D->w_pts)[min(tk.w0 + TILE, n_w) - 1];
+  const T w_abs = fmax(fabs(w_first), fabs(w_last));  // max |w| over the block (w_pts increase)
   const bool terminal = D->terminal != 0;
   const bool taste = D->has_taste != 0;
   // a stateful continuation needs a next-state grid with at least one point
@@ -364,10 +459,11 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   // LDS carve (the host sizes the launch for the largest g_n and n_constr)
   const size_t g_bytes = align16(size_t(g_n > 0 ? g_n : 1) * sizeof(T));
   T* ev = reinterpret_cast<T*>(smem);
-  T* gp = reinterpret_cast<T*>(smem + g_bytes);
-  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
-  T* his = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
-  const size_t con_ofs = 2 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
+  T* gp = reinterpret_cast<T*>(smem + g_bytes);  // irregular grid points, or
+  Pair<T>* pairs = reinterpret_cast<Pair<T>*>(smem + g_bytes);  // the folded row (fast variant)
+  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 3 * g_bytes);
+  T* his = reinterpret_cast<T*>(smem + 3 * g_bytes + CH * sizeof(Act<T>));
+  const size_t con_ofs = 3 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
   Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);
   T* red_v = reinterpret_cast<T*>(smem + con_ofs + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
@@ -378,6 +474,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   L.his = his;
   L.con = con;
   L.ev = ev;
+  L.pairs = pairs;
   L.D = D;
   L.w = w;
   L.valid = valid;
@@ -420,8 +517,14 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   for (int d = 0; d < n_d; ++d) {
     T cs = T(0);
     bool finite_row = true;
+    bool row_small = false;  // folded row built and within the folded loop's bounds
     if (!terminal) {
       __syncthreads();  // previous readers of ev are done
+      for (int s = 0; s < D->n_scalar; ++s) {
+        const T m = static_cast<const T*>(D->ms[s])[int64_t(sd) * n_d + d];
+        const T v = static_cast<const T*>(D->Vs[s])[0];
+        cs += zero_safe(m, v);
+      }
       // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k]   (accumulated in T)
       int local_nonfinite = 0;
       for (int k = tid; k < g_n; k += kBlock) {
@@ -439,10 +542,19 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         local_nonfinite |= !is_finite(acc);
       }
       finite_row = __syncthreads_or(local_nonfinite) == 0;
-      for (int s = 0; s < D->n_scalar; ++s) {
-        const T m = static_cast<const T*>(D->ms[s])[int64_t(sd) * n_d + d];
-        const T v = static_cast<const T*>(D->Vs[s])[0];
-        cs += zero_safe(m, v);
+      if (FAST && mode == kModeInterp && finite_row && g_n >= 2) {
+        // folded row: pairs (s_k, s_{k+1} - s_k) of s = beta * (EV + cs)
+        int big = 0;
+        for (int k = tid; k < g_n - 1; k += kBlock) {
+          const T s0 = L.beta * (ev[k] + cs);
+          const T s1 = L.beta * (ev[k + 1] + cs);
+          Pair<T> p;
+          p.v = s0;
+          p.dv = s1 - s0;
+          pairs[k] = p;
+          big |= !(fabs(p.v) <= T(1e30)) | !(fabs(p.dv) <= T(1e30));
+        }
+        row_small = __syncthreads_or(big) == 0;
       }
     }
     L.cs = cs;
@@ -450,6 +562,10 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     L.Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
     L.Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
     L.Bg = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
+    // the common cases run the folded loop: interpolation with the grid
+    // folded into A, B and the EV row scaled, or a scalar continuation
+    const bool folded = FAST && ((mode == kModeInterp && finite_row && row_small) || mode == kModeScalar);
+    const T bcs = L.beta * cs;
 
     for (int c0 = 0; c0 < n_c; c0 += CH) {
       const int cn = min(CH, n_c - c0);
@@ -460,6 +576,47 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         const T* fc = static_cast<const T*>(D->Fchunk) + int64_t(sd) * D->Fc_ssd +
                       (int64_t(d) * ((n_c + CH - 1) / CH) + c0 / CH) * 2;
         if (fc[0] > w_last || fc[1] < w_first) continue;
+      }
+      if constexpr (FAST) {
+        if (folded) {
+          __syncthreads();  // previous chunk consumed
+          const int64_t fi = int64_t(sd) * D->Fi_ssd + int64_t(d) * n_c + c0;
+          int need_lo = 0, need_hi = 0, unsafe = 0;
+          for (int i = tid; i < cn; i += kBlock) {
+            Act<T> a;
+            if (interp) {
+              const T A = L.Ag[c0 + i];
+              const T B = L.Bg[c0 + i] - g.start;
+              if constexpr (std::is_same<T, double>::value) {
+                a.A = A / g.step;
+                a.B = B / g.step;
+              } else {
+                a.A = A * g.inv_step;
+                a.B = B * g.inv_step;
+              }
+              a.U = L.Ug[c0 + i];
+              // |coord| over the block's states (w_pts increase) and U not NaN
+              unsafe |= !(fabs(a.A) * w_abs + fabs(a.B) <= T(4194304)) | (a.U != a.U);
+            } else {
+              a.A = a.B = T(0);
+              a.U = L.Ug[c0 + i] + bcs;  // Q of a scalar continuation
+              unsafe |= a.U != a.U;
+            }
+            a.lo = static_cast<const T*>(D->Flo)[fi + i];
+            const T hi = static_cast<const T*>(D->Fhi)[fi + i];
+            act[i] = a;
+            his[i] = hi;
+            need_lo |= !(a.lo == T(-INFINITY));
+            need_hi |= !(hi == T(INFINITY));
+          }
+          const bool lo = __syncthreads_or(need_lo) != 0;
+          const bool hi = __syncthreads_or(need_hi) != 0;
+          if (__syncthreads_or(unsafe) == 0) {
+            folded_dispatch<T, SPLIT>(L, interp, lo, hi, run);
+            continue;
+          }
+          // otherwise the staged chunk below replaces the folded one
+        }
       }
       if constexpr (!WDEP) {
         __syncthreads();  // previous chunk consumed
@@ -572,7 +729,7 @@ extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
   const size_t his = (lcm::kActChunk * sz + 15) & ~size_t(15);
-  return 2 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
+  return 3 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
          lcm::kBlock * (sz + 8) + 64;
 }
 
