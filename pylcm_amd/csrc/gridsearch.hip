@@ -46,6 +46,11 @@ constexpr int kVarFast = 0;
 constexpr int kVarStaged = 1;
 constexpr int kVarWdep = 2;
 
+// The fast variant keeps the EV row as (value, slope) pairs, so node k of the
+// row sits at element 2k; the other variants keep it contiguous.
+template <int VAR>
+constexpr int kEvStride = VAR == kVarFast ? 2 : 1;
+
 // Continuation kinds of a descriptor (block-uniform).
 constexpr int kModeInterp = 0;    // stateful targets: interpolate EV
 constexpr int kModeTerminal = 1;  // terminal regime: Q = U
@@ -103,16 +108,16 @@ __device__ __forceinline__ double clamp_index(double x, double hi) {
 
 // Linear interpolation with linear extrapolation: lower index clipped to
 // [0, n-2] (ndimage.py:102-109); with FINITE false a zero corner weight drops
-// its term (the row holds -inf somewhere).
-template <typename T, bool FINITE>
+// its term (the row holds -inf somewhere). Node k of the row is row[k * ES].
+template <typename T, bool FINITE, int ES>
 __device__ __forceinline__ T interp_row(const T* row, int n, T coord) {
   if (n < 2) return row[0];
   const T fl = clamp_index(floor(coord), T(n - 2));
   const int lo = static_cast<int>(fl);
   const T t = coord - fl;
   const T w0 = T(1) - t;
-  const T v0 = row[lo];
-  const T v1 = row[lo + 1];
+  const T v0 = row[lo * ES];
+  const T v1 = row[(lo + 1) * ES];
   if constexpr (FINITE) return w0 * v0 + t * v1;
   return zero_safe(w0, v0) + zero_safe(t, v1);
 }
@@ -226,7 +231,7 @@ __device__ __forceinline__ T staged_q(const LoopArgs<T>& L, const Act<T>& a) {
     T coord;
     if constexpr (VAR == kVarFast) coord = lin_coordinate(L.g, x);
     else coord = coordinate(L.g, x);
-    return a.U + L.beta * (L.cs + interp_row<T, FINITE>(L.ev, L.g_n, coord));
+    return a.U + L.beta * (L.cs + interp_row<T, FINITE, kEvStride<VAR>>(L.ev, L.g_n, coord));
   } else if constexpr (MODE == kModeTerminal) {
     return a.U;
   } else {
@@ -259,11 +264,11 @@ __device__ __forceinline__ void staged_loop(const LoopArgs<T>& L, Running<T>& ru
 // Folded loop of the fast variant for the common case (interval feasibility,
 // linear next grid, finite EV row with at least two points). Staging folded
 // the grid into each action, so the coordinate of the next state is one fma,
-// coord = A' * w + B' with A' = A / step and B' = (B - start) / step, and the
-// EV row is held as pairs (s_k, s_{k+1} - s_k) of s = beta * (EV + cs), so
-// Q = U + s_lo + t * slope. The block runs this loop only on chunks whose
-// magnitudes (|coord| <= 2^22, |s|, |slope| <= 1e30, U not NaN) rule out
-// overflow and NaN; the other chunks take the staged loop. LO / HI drop the
+// coord = A' * w + B' with A' = A / step and B' = (B - start) / step, and
+// U' = U + beta * cs; the EV row is held as pairs (EV_k, EV_{k+1} - EV_k), so
+// Q = U' + beta * (EV_lo + t * slope). The block runs this loop only on chunks
+// whose magnitudes (|coord| <= 2^22, |EV|, |slope| <= 1e30, U' not NaN) rule
+// out overflow and NaN; the other chunks take the staged loop. LO / HI drop the
 // bound checks a chunk does not need (all lower bounds -inf / all upper
 // bounds +inf). Four actions per iteration give the scheduler independent LDS
 // reads. The update is `offer`'s first-argmax rule, with NaN as the running
@@ -283,12 +288,13 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     if constexpr (HI) f = f & (w <= L.his[i]);
     return f;
   };
+  const T beta = L.beta;
   auto q_of = [&](const Act<T>& a) -> T {
     if constexpr (!INTERP) return a.U;  // staged as U + beta * cs
     const T coord = a.A * w + a.B;
     const T fl = clamp_index(T(floor(coord)), top);
     const Pair<T> p = pr[static_cast<int>(fl)];
-    return a.U + (p.v + (coord - fl) * p.dv);
+    return a.U + beta * (p.v + (coord - fl) * p.dv);
   };
   auto update = [&](T q, bool f, int c) {
     const bool take = f & !(q <= best);
@@ -394,7 +400,7 @@ __device__ __forceinline__ void action_loop(const LoopArgs<T>& L, Running<T>& ru
       T coord;
       if constexpr (VAR == kVarFast) coord = lin_coordinate(L.g, x);
       else coord = coordinate(L.g, x);
-      q = Uc + L.beta * (L.cs + interp_row<T, FINITE>(L.ev, L.g_n, coord));
+      q = Uc + L.beta * (L.cs + interp_row<T, FINITE, kEvStride<VAR>>(L.ev, L.g_n, coord));
     } else if constexpr (MODE == kModeTerminal) {
       q = Uc;
     } else {
@@ -458,12 +464,14 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
 
   // LDS carve (the host sizes the launch for the largest g_n and n_constr)
   const size_t g_bytes = align16(size_t(g_n > 0 ? g_n : 1) * sizeof(T));
+  // EV row: contiguous, or in the fast variant the .v members of the pairs
+  constexpr int ES = kEvStride<VAR>;
   T* ev = reinterpret_cast<T*>(smem);
-  T* gp = reinterpret_cast<T*>(smem + g_bytes);  // irregular grid points, or
-  Pair<T>* pairs = reinterpret_cast<Pair<T>*>(smem + g_bytes);  // the folded row (fast variant)
-  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 3 * g_bytes);
-  T* his = reinterpret_cast<T*>(smem + 3 * g_bytes + CH * sizeof(Act<T>));
-  const size_t con_ofs = 3 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
+  Pair<T>* pairs = reinterpret_cast<Pair<T>*>(smem);
+  T* gp = reinterpret_cast<T*>(smem + g_bytes);  // irregular grid points (not the fast variant)
+  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
+  T* his = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
+  const size_t con_ofs = 2 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
   Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);
   T* red_v = reinterpret_cast<T*>(smem + con_ofs + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
@@ -538,21 +546,18 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
             if (m != T(0)) acc += m * Vr[int64_t(j) * g_n + k];
           }
         }
-        ev[k] = acc;
+        ev[k * ES] = acc;
         local_nonfinite |= !is_finite(acc);
       }
       finite_row = __syncthreads_or(local_nonfinite) == 0;
       if (FAST && mode == kModeInterp && finite_row && g_n >= 2) {
-        // folded row: pairs (s_k, s_{k+1} - s_k) of s = beta * (EV + cs)
+        // slopes of the folded row; the values stay as the other loops read them
         int big = 0;
-        for (int k = tid; k < g_n - 1; k += kBlock) {
-          const T s0 = L.beta * (ev[k] + cs);
-          const T s1 = L.beta * (ev[k + 1] + cs);
-          Pair<T> p;
-          p.v = s0;
-          p.dv = s1 - s0;
-          pairs[k] = p;
-          big |= !(fabs(p.v) <= T(1e30)) | !(fabs(p.dv) <= T(1e30));
+        for (int k = tid; k < g_n; k += kBlock) {
+          const T v = pairs[k].v;
+          const T dv = k + 1 < g_n ? pairs[k + 1].v - v : T(0);
+          pairs[k].dv = dv;
+          big |= !(fabs(v) <= T(1e30)) | !(fabs(dv) <= T(1e30));
         }
         row_small = __syncthreads_or(big) == 0;
       }
@@ -594,8 +599,8 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
                 a.A = A * g.inv_step;
                 a.B = B * g.inv_step;
               }
-              a.U = L.Ug[c0 + i];
-              // |coord| over the block's states (w_pts increase) and U not NaN
+              a.U = L.Ug[c0 + i] + bcs;
+              // |coord| over the block's states (w_pts increase) and U' not NaN
               unsafe |= !(fabs(a.A) * w_abs + fabs(a.B) <= T(4194304)) | (a.U != a.U);
             } else {
               a.A = a.B = T(0);
@@ -729,7 +734,7 @@ extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
   const size_t his = (lcm::kActChunk * sz + 15) & ~size_t(15);
-  return 3 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
+  return 2 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
          lcm::kBlock * (sz + 8) + 64;
 }
 

@@ -169,3 +169,45 @@ def nonaffine_model(n_periods):
     regs["retirement"] = regs["retirement"].replace(
         functions={"utility": u_r}, state_transitions={"wealth": nw})
     return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
+
+
+def nan_infeasible_utility_model(n_periods):
+    """Utility is NaN for consumption above 420, which no wealth node (at most
+    400) can afford: the NaN sits in staged action chunks but is never feasible."""
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=300, n_consumption=300, consumption_range=(1.0, 500.0))
+
+    def slack(consumption):
+        return np.log(np.where(consumption < 420.0, 420.0 - consumption, np.nan))
+
+    def u_w(consumption, is_working, disutility_of_work):
+        return np.log(consumption) + 0.01 * slack(consumption) - np.where(is_working, disutility_of_work, 0.0)
+
+    def u_r(consumption):
+        return np.log(consumption) + 0.01 * slack(consumption)
+
+    regs = dict(m.user_regimes)
+    regs["working_life"] = regs["working_life"].replace(
+        functions={"utility": u_w, "labor_income": ex.labor_income, "is_working": ex.is_working})
+    regs["retirement"] = regs["retirement"].replace(functions={"utility": u_r})
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
+
+
+def scaled_utility_model(n_periods, scale):
+    """Log utility times `scale` (value functions of magnitude ~scale)."""
+    from pylcm_amd import examples as ex
+
+    m = ex.retirement_model(n_periods, n_wealth=90, n_consumption=120, income_nodes=3)
+
+    def u_w(consumption, is_working, disutility_of_work):
+        return scale * (np.log(consumption) - np.where(is_working, disutility_of_work, 0.0))
+
+    def u_r(consumption):
+        return scale * np.log(consumption)
+
+    regs = dict(m.user_regimes)
+    regs["working_life"] = regs["working_life"].replace(
+        functions={"utility": u_w, "labor_income": ex.labor_income, "is_working": ex.is_working})
+    regs["retirement"] = regs["retirement"].replace(functions={"utility": u_r})
+    return Model(regimes=regs, ages=m.ages, regime_id_class=m.regime_id_class)
