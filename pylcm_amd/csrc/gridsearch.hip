@@ -198,9 +198,11 @@ struct LoopArgs {
   const T* Ag;
   const T* Bg;
   const Pair<T>* pairs;  // folded EV row (fast variant)
+  const T* pm;           // prefix max of the staged lower bounds (fast variant)
   const LcmGsDesc* D;
   Grid1D<T> g;
   T w, beta, cs;
+  T w_wave;  // smallest state of this wave (its lane 0; states increase along the lanes)
   int n_constr, strict_mask, g_n, n_c, d, sd, wr, c0, cn, grp;
   bool valid;
 };
@@ -301,7 +303,32 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     best = take ? q : best;
     idx = take ? c : idx;
   };
+  if (__builtin_amdgcn_ballot_w64(L.valid) == 0) return;  // no state of this wave is real
   int i = L.grp;
+  if constexpr (LO && !HI) {
+    // Actions [0, K) have every lower bound up to theirs at most the wave's
+    // smallest state, so every lane can take them: no bound checks there.
+    int a = 0, b = L.cn;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (__builtin_amdgcn_readfirstlane(int(L.pm[mid] <= L.w_wave))) a = mid + 1;
+      else b = mid;
+    }
+    for (; i + 3 * SPLIT < a; i += 4 * SPLIT) {
+      const Act<T> a0 = L.act[i];
+      const Act<T> a1 = L.act[i + SPLIT];
+      const Act<T> a2 = L.act[i + 2 * SPLIT];
+      const Act<T> a3 = L.act[i + 3 * SPLIT];
+      const T q0 = q_of(a0);
+      const T q1 = q_of(a1);
+      const T q2 = q_of(a2);
+      const T q3 = q_of(a3);
+      update(q0, L.valid, base + i);
+      update(q1, L.valid, base + i + SPLIT);
+      update(q2, L.valid, base + i + 2 * SPLIT);
+      update(q3, L.valid, base + i + 3 * SPLIT);
+    }
+  }
   for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
     const Act<T> a0 = L.act[i];
     const Act<T> a1 = L.act[i + SPLIT];
@@ -454,6 +481,8 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const T w_first = static_cast<const T*>(D->w_pts)[tk.w0];
   const T w_last = static_cast<const T*>(D->w_pts)[min(tk.w0 + TILE, n_w) - 1];
   const T w_abs = fmax(fabs(w_first), fabs(w_last));  // max |w| over the block (w_pts increase)
+  const int ls0 = (tid & ~63) % TILE;  // state of this wave's lane 0
+  const T w_wave = static_cast<const T*>(D->w_pts)[min(tk.w0 + ls0, n_w - 1)];
   const bool terminal = D->terminal != 0;
   const bool taste = D->has_taste != 0;
   // a stateful continuation needs a next-state grid with at least one point
@@ -472,8 +501,11 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
   T* his = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
   const size_t con_ofs = 2 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
-  Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);
-  T* red_v = reinterpret_cast<T*>(smem + con_ofs + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
+  Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);  // constraint rows (staged variant)
+  T* pm = reinterpret_cast<T*>(smem + con_ofs);              // fast variant: prefix max of lo
+  T* wtot = reinterpret_cast<T*>(smem + con_ofs + align16(CH * sizeof(T)));  // its per-wave maxima
+  const size_t pm_bytes = FAST ? align16(CH * sizeof(T)) + 64 : 0;
+  T* red_v = reinterpret_cast<T*>(smem + con_ofs + pm_bytes + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
   int* red_f = red_i + kBlock;
 
@@ -483,6 +515,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   L.con = con;
   L.ev = ev;
   L.pairs = pairs;
+  L.pm = pm;
   L.D = D;
   L.w = w;
   L.valid = valid;
@@ -494,6 +527,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   L.wr = wr;
   L.grp = grp;
   L.beta = T(D->beta);
+  L.w_wave = w_wave;
 
   Grid1D<T>& g = L.g;
   g.kind = D->g_kind;
@@ -587,6 +621,8 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
           __syncthreads();  // previous chunk consumed
           const int64_t fi = int64_t(sd) * D->Fi_ssd + int64_t(d) * n_c + c0;
           int need_lo = 0, need_hi = 0, unsafe = 0;
+          T lo_me = T(-INFINITY);  // this thread's lower bound (one action per thread)
+          static_assert(CH == kBlock, "one staged action per thread");
           for (int i = tid; i < cn; i += kBlock) {
             Act<T> a;
             if (interp) {
@@ -613,8 +649,20 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
             his[i] = hi;
             need_lo |= !(a.lo == T(-INFINITY));
             need_hi |= !(hi == T(INFINITY));
+            lo_me = a.lo == a.lo ? a.lo : T(INFINITY);  // a NaN bound is never met
           }
+          // inclusive prefix max of the lower bounds: within the wave by
+          // shuffles, then across waves through LDS
+          const int lane = tid & 63;
+#pragma unroll
+          for (int off = 1; off < 64; off <<= 1) {
+            const T y = __shfl_up(lo_me, off);
+            if (lane >= off) lo_me = fmax(lo_me, y);
+          }
+          if (lane == 63) wtot[tid >> 6] = lo_me;
           const bool lo = __syncthreads_or(need_lo) != 0;
+          for (int k = 0; k < (tid >> 6); ++k) lo_me = fmax(lo_me, wtot[k]);
+          if (tid < cn) pm[tid] = lo_me;
           const bool hi = __syncthreads_or(need_hi) != 0;
           if (__syncthreads_or(unsafe) == 0) {
             folded_dispatch<T, SPLIT>(L, interp, lo, hi, run);
@@ -734,7 +782,7 @@ extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
   const size_t his = (lcm::kActChunk * sz + 15) & ~size_t(15);
-  return 2 * g + lcm::kActChunk * 4 * sz + his + size_t(max_constr) * lcm::kActChunk * 2 * sz +
+  return 2 * g + lcm::kActChunk * 4 * sz + 2 * his + 64 + size_t(max_constr) * lcm::kActChunk * 2 * sz +
          lcm::kBlock * (sz + 8) + 64;
 }
 
