@@ -154,6 +154,109 @@ def dcegm_leg(args):
     return out
 
 
+def simulation_leg(args):
+    """Forward simulation of the stochastic workload (f32, T=40, 500 x 500,
+    7 Rouwenhorst nodes): N agents, w0 ~ exp(log 10 + 0.5 N(0,1)), the income
+    node drawn from Binomial(6, 1/2), Philox4x32-10 streams per (agent,
+    period, draw). The V tables come from the GPU solve and stay resident;
+    the timed part is the EV rows plus the 40 period launches."""
+    from pylcm_amd import _native as nat
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import GsProgram
+    from pylcm_amd._simulate import SimProgram, _initial_conditions
+
+    dt = np.float32
+    m, p = ex.stochastic_workload()
+    T = m.n_periods
+    prog = GsProgram(m, [p], dt)
+    prog.run(use_graph=False)
+    n = args.sim_agents
+    rng = np.random.default_rng(20261203)
+    from pylcm_amd._build import resolve_grids
+
+    rg = resolve_grids(m.layouts, p, m.fixed_params, dt)
+    nodes = np.asarray(rg.proc[("working_life", "income")][0], dtype=np.float64)
+    w0 = np.exp(np.log(10.0) + 0.5 * rng.standard_normal(n))
+    z0 = nodes[rng.binomial(6, 0.5, n)]
+    ic_user = {"wealth": w0, "income": z0, "regime_id": np.zeros(n, dtype=int)}
+    ic = _initial_conditions(m, ic_user)
+    seed = 20261203
+    sim = SimProgram(m, p, prog)
+    t0 = time.perf_counter()
+    sim.prepare(ic, seed)
+    prep_s = time.perf_counter() - t0
+    sim.launch()
+    prog.ctx.sync()  # warm-up
+    reps = max(1, args.steps)
+    ev_ms, per_ms, tot_ms = [], [], []
+    for _ in range(reps):
+        evs = ((nat.Event(), nat.Event()), [(nat.Event(), nat.Event()) for _ in range(T)])
+        sim.launch(events=evs)
+        prog.ctx.sync()
+        ev_ms.append(evs[0][0].elapsed_ms(evs[0][1]))
+        per_ms.append([a.elapsed_ms(b) for a, b in evs[1]])
+        tot_ms.append(evs[0][0].elapsed_ms(evs[1][-1][1]))
+    res = sim.collect(ic)
+    raw = res.raw
+    tot = float(np.median(tot_ms))
+    agent_periods = int((raw["action"] >= 0).sum())
+    # algorithmic work: n_actions x 10 flops per simulated agent-period
+    names = list(m.regime_names)
+    n_act = np.array([m.layouts[r].n_d * m.layouts[r].n_c for r in names], dtype=np.float64)
+    simulated = raw["action"] >= 0
+    flops = 10.0 * float(n_act[raw["regime"][simulated]].sum())
+    per = np.median(np.array(per_ms), axis=0)
+    sim_ms = float(per.sum())
+    # EV rows: reads M and V_{t+1}, writes EV, per (period, regime)
+    ev_bytes = 0
+    for (t, r), pt in prog.problems[0].items():
+        lay = m.layouts[r]
+        if pt.terminal or not pt.targets:
+            continue
+        g_n = len(rg.w_pts[pt.next_grid])
+        for _, M in pt.targets:
+            ev_bytes += 4 * (M.size + M.shape[-1] * g_n)
+        ev_bytes += 4 * lay.n_sd * lay.n_d * g_n
+    ev = float(np.median(ev_ms))
+    out = {
+        "workload": f"{n} agents x {T} periods of the stochastic retirement model (f32, 500 x 500, 7 Rouwenhorst "
+                    "nodes), V resident from the GPU solve, Philox4x32-10 draws",
+        "agents": n, "periods": T,
+        "ms_per_simulation": round(tot, 3),
+        "agent_periods_per_s": round(agent_periods / (tot * 1e-3), 1),
+        "host_prepare_s": round(prep_s, 3),
+        "roofline": {"kernel": "sim_kernel<float> (+ per-period partition kernels)", "bound": "valu",
+                     "achieved": round(flops / (sim_ms * 1e-3) / 1e12, 3), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(flops / (sim_ms * 1e-3) / 1e12 / 157.3, 4), "flop_per_agent_action": 10,
+                     "period_launch_ms_median": round(float(np.median(per)), 4)},
+        "ev_kernel": {"ms": round(ev, 4), "bytes": ev_bytes, "achieved_GBs": round(ev_bytes / (ev * 1e-3) / 1e9, 2),
+                      "peak_GBs": 8000, "bound": "hbm"},
+    }
+    log(f"simulation: {n} agents x {T} periods in {tot:.2f} ms ({out['agent_periods_per_s']:.3g} agent-periods/s); "
+        f"sim kernels {sim_ms:.2f} ms, {out['roofline']['achieved']} TFLOP/s; EV {ev:.3f} ms")
+    if not args.no_cpu:
+        from oracle import simulate as osim
+
+        k = min(args.sim_cpu_agents, n)
+        prog.fetch()
+        Vg = {t: {r: v for r, (v, _) in rr.items()} for t, rr in prog.results(0).items()}
+        sub = {key: val[:k] for key, val in ic_user.items()}
+        t0 = time.perf_counter()
+        exp = osim.simulate(m, p, Vg, sub, seed, grid_dtype=dt)
+        cpu_s = time.perf_counter() - t0
+        cpu_ap = int((exp["action"] >= 0).sum())
+        same = simulated[:, :k] == (exp["action"] >= 0)
+        both = simulated[:, :k] & (exp["action"] >= 0)
+        act_mism = int((raw["action"][:, :k][both] != exp["action"][both]).sum())
+        out["cpu_baseline"] = {"value": round(cpu_ap / cpu_s, 1), "unit": "agent-periods/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"first {k} agents, NumPy FP64 restatement oracle/simulate.py, {cpu_s:.1f}s"}
+        out["parity_sample"] = {"agents": k, "same_simulated_mask": bool(same.all()),
+                                "action_mismatches": act_mism, "agent_periods": int(both.sum())}
+        log(f"simulation cpu baseline: {out['cpu_baseline']['value']} agent-periods/s; parity {out['parity_sample']}")
+    return out
+
+
 def serving_leg(args):
     """Poisson arrivals (rate per second) for a fixed window into the
     continuous-batching server; requests mix the deterministic and the AR(1)
@@ -228,6 +331,9 @@ def main():
     ap.add_argument("--copy-calib-mib", type=int, default=256)
     ap.add_argument("--no-dcegm", action="store_true")
     ap.add_argument("--dcegm-periods", type=int, default=50)
+    ap.add_argument("--no-sim", action="store_true")
+    ap.add_argument("--sim-agents", type=int, default=100_000)
+    ap.add_argument("--sim-cpu-agents", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
@@ -377,6 +483,7 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3f} solves/s on {threads} threads; parity {parity}")
 
     dcegm = None if args.no_dcegm else dcegm_leg(args)
+    simulation = None if args.no_sim else simulation_leg(args)
     serving = serving_leg(args) if args.serve_seconds > 0 else None
 
     key = f"draws={args.draws} periods={T} tile={prog.tile} dtype=f32"
@@ -429,6 +536,7 @@ def main():
         "parity": parity,
         "host_lowering_s": round(lower_s, 2),
         "dcegm": dcegm,
+        "simulation": simulation,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

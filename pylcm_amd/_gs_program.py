@@ -433,18 +433,7 @@ class GsProgram:
     def relocatable_records(self):
         """(records for base address 0, {field: mask of its pointer entries}):
         `records_at(base)` turns them into the descriptors for any base."""
-        saved = getattr(self, "base", None)
-        self.base = 0
-        D0 = self._desc_records(self._descs)
-        self.base = 1
-        D1 = self._desc_records(self._descs)
-        self.base = saved
-        masks = {}
-        for f in D0.dtype.names:
-            m = D1[f] != D0[f]
-            if np.any(m):
-                masks[f] = m
-        return D0, masks
+        return self._desc_records(self._descs, base=0, with_masks=True)
 
     @staticmethod
     def records_at(D0, masks, base: int):
@@ -453,40 +442,51 @@ class GsProgram:
             D[f][m] += np.uint64(base)
         return D
 
-    def _desc_records(self, descs):
-        """Descriptor records, filled column by column."""
-        base = self.base
+    _PTR_FIELDS = ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi", "Fchunk")
+    _VAL_FIELDS = ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
+                   "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
+                   "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "Fc_ssd", "use_interval")
+    # (record field, descriptor key, slots, pointer?, element of a (ssd, sw) pair)
+    _LIST_FIELDS = (("Fa", "Fa", nat.MAX_CONSTR, True, None), ("Fb", "Fb", nat.MAX_CONSTR, True, None),
+                    ("Fa_ssd", "Fa_s", nat.MAX_CONSTR, False, 0), ("Fa_sw", "Fa_s", nat.MAX_CONSTR, False, 1),
+                    ("Fb_ssd", "Fb_s", nat.MAX_CONSTR, False, 0), ("Fb_sw", "Fb_s", nat.MAX_CONSTR, False, 1),
+                    ("Vn", "Vn", nat.MAX_TARGETS, True, None), ("M", "M", nat.MAX_TARGETS, True, None),
+                    ("n_sd_next", "nj", nat.MAX_TARGETS, False, None),
+                    ("Vs", "Vs", nat.MAX_SCALAR, True, None), ("ms", "ms", nat.MAX_SCALAR, True, None))
+
+    def _desc_records(self, descs, base=None, with_masks=False):
+        """Descriptor records, filled column by column; pointers are arena
+        offsets plus `base` (default: the program's). With `with_masks`, also
+        returns {field: mask of its pointer entries}."""
+        base = self.base if base is None else base
         n = len(descs)
         D = np.zeros(n, dtype=DESC_DTYPE)
-        for f in ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi", "Fchunk"):
-            D[f] = base + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
-        for f in ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
-                  "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
-                  "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "Fc_ssd", "use_interval"):
+        masks = {}
+        for f in self._PTR_FIELDS:
+            D[f] = np.uint64(base) + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
+            if with_masks:
+                masks[f] = np.ones(n, dtype=bool)
+        for f in self._VAL_FIELDS:
             D[f] = np.array([d[f] for d in descs], dtype=D.dtype[f])
         D["n_targets"] = np.array([len(d["Vn"]) for d in descs], dtype=np.int32)
         D["n_scalar"] = np.array([len(d["Vs"]) for d in descs], dtype=np.int32)
-
-        def fill(field, key, k, transform):
-            rows = [i for i, d in enumerate(descs) if len(d[key]) > k]
-            if rows:
-                D[field][rows, k] = [transform(descs[i][key][k]) for i in rows]
-
-        for k in range(nat.MAX_CONSTR):
-            fill("Fa", "Fa", k, lambda v: base + v)
-            fill("Fb", "Fb", k, lambda v: base + v)
-            fill("Fa_ssd", "Fa_s", k, lambda v: v[0])
-            fill("Fa_sw", "Fa_s", k, lambda v: v[1])
-            fill("Fb_ssd", "Fb_s", k, lambda v: v[0])
-            fill("Fb_sw", "Fb_s", k, lambda v: v[1])
-        for k in range(nat.MAX_TARGETS):
-            fill("Vn", "Vn", k, lambda v: base + v)
-            fill("M", "M", k, lambda v: base + v)
-            fill("n_sd_next", "nj", k, lambda v: v)
-        for k in range(nat.MAX_SCALAR):
-            fill("Vs", "Vs", k, lambda v: base + v)
-            fill("ms", "ms", k, lambda v: base + v)
-        return D
+        for field, key, k, ptr, part in self._LIST_FIELDS:
+            lens = np.fromiter((len(d[key]) for d in descs), dtype=np.int64, count=n)
+            if not lens.any():
+                continue
+            pad = [0] * k
+            if part is None:
+                rows = [(list(d[key]) + pad)[:k] for d in descs]
+            else:
+                rows = [([v[part] for v in d[key]] + pad)[:k] for d in descs]
+            vals = np.array(rows, dtype=D.dtype[field].base)
+            present = np.arange(k)[None, :] < lens[:, None]
+            if ptr:
+                vals = np.where(present, vals + np.uint64(base), vals).astype(np.uint64)
+                if with_masks:
+                    masks[field] = present
+            D[field] = vals
+        return (D, masks) if with_masks else D
 
     # -- execution -------------------------------------------------------------
     def _launch_step(self, s: int):

@@ -390,6 +390,14 @@ class SimProgram:
                 raise UnsupportedModelError("simulation of taste-shock regimes is not implemented")
 
     def run(self, ic, seed):
+        self.prepare(ic, seed)
+        self.launch()
+        self.prog.ctx.sync()
+        return self.collect(ic)
+
+    def prepare(self, ic, seed):
+        """Allocate the agent panels, lower every period's tables and upload
+        them with the descriptors; `launch` then enqueues the simulation."""
         model, dt, prog = self.model, self.dt, self.prog
         T = model.n_periods
         n = ic["_n"]
@@ -511,14 +519,39 @@ class SimProgram:
         ar.parts.append((sim_desc_off, SD.tobytes()))
         ar.size = max(ar.size, sim_desc_off + SD.nbytes)
         nat.h2d(ctx, b2, ar.image())
-        # -- run ------------------------------------------------------------------
+        self._st = dict(T=T, n=n, R=R, seed=seed, base=base, off=off, b2=b2, ev_desc_off=ev_desc_off,
+                        sim_desc_off=sim_desc_off, ev_keys=ev_keys, lds_max=lds_max,
+                        sim_desc_size=SD.dtype.itemsize, keep=keep_tables)
+        # agent panels are rewritten by `launch`; keep the initial state to restore
+        self._init_state = [(off[side][k], arr) for side, k, arr in (
+            ("cur", "w", w0.astype(dt)), ("cur", "z", z0.astype(dt)), ("cur", "sdd", sdd0),
+            ("cur", "regime", reg0), ("cur", "start", ic["_start"]), ("nxt", "start", ic["_start"]))]
+        self._launched = False
+
+    def launch(self, events=None):
+        """Enqueue the EV rows and the T period launches on the solve
+        program's stream (no host sync). With `events` = (ev_pair, [pair per
+        period]) of `nat.Event`, records them around the EV launch and each
+        period."""
+        st, model, dt, prog = self._st, self.model, self.dt, self.prog
+        ctx = prog.ctx
+        T, n, R, seed, base, off, b2 = (st[k] for k in ("T", "n", "R", "seed", "base", "off", "b2"))
+        if self._launched:
+            # a repeated run starts from the same initial panel
+            for o, arr in self._init_state:
+                nat.h2d(ctx, base + o, arr)
+        self._launched = True
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
         # EV rows for every (period, regime) with a stateful continuation
-        if ev_keys:
-            max_rows = max(model.layouts[r].n_sd * model.layouts[r].n_d for _, r in ev_keys)
-            nat._check(L.lcm_ev_launch(ctx.handle, code, C.c_void_p(b2 + ev_desc_off), len(ev_keys), max_rows),
-                       "lcm_ev_launch")
+        if events:
+            events[0][0].record(ctx)
+        if st["ev_keys"]:
+            max_rows = max(model.layouts[r].n_sd * model.layouts[r].n_d for _, r in st["ev_keys"])
+            nat._check(L.lcm_ev_launch(ctx.handle, code, C.c_void_p(b2 + st["ev_desc_off"]), len(st["ev_keys"]),
+                                       max_rows), "lcm_ev_launch")
+        if events:
+            events[0][1].record(ctx)
 
         def agents(side):
             o = off[side]
@@ -528,13 +561,22 @@ class SimProgram:
         O = nat.LcmSimOut(*(base + off["out"][k] for k in ("regime", "action", "value", "w", "z", "sdd")))
         for t in range(T):
             cur, nxt = A[t % 2], A[(t + 1) % 2]
-            st = L.lcm_sim_period_ex(ctx.handle, code, C.c_void_p(b2 + sim_desc_off + t * R * SD.dtype.itemsize),
+            if events:
+                events[1][t][0].record(ctx)
+            rc = L.lcm_sim_period_ex(ctx.handle, code, C.c_void_p(b2 + st["sim_desc_off"] + t * R * st["sim_desc_size"]),
                                      R, C.byref(cur), C.byref(nxt), C.byref(O), n, t, C.c_uint64(seed),
-                                     C.c_size_t(lds_max), C.c_void_p(base + off["counts"]),
+                                     C.c_size_t(st["lds_max"]), C.c_void_p(base + off["counts"]),
                                      C.c_void_p(base + off["order"]))
-            nat._check(st, f"lcm_sim_period (period {t})")
-        ctx.sync()
-        # -- download -------------------------------------------------------------
+            nat._check(rc, f"lcm_sim_period (period {t})")
+            if events:
+                events[1][t][1].record(ctx)
+
+    def collect(self, ic):
+        """Download the panel (after the stream has finished) and build the result."""
+        st, model, dt, ctx = self._st, self.model, self.dt, self.prog.ctx
+        T, n, base, off = st["T"], st["n"], st["base"], st["off"]
+        names = list(model.regime_names)
+
         def get(name, dtype_):
             arr = np.zeros(T * n, dtype=dtype_)
             nat.d2h(ctx, base + off["out"][name], arr)

@@ -10,6 +10,8 @@ namespace lcm {
 
 constexpr int kSimBlock = 256;
 constexpr int kSimChunk = 128;  // actions staged in LDS at a time
+constexpr int kSimLanes = 8;    // lanes per agent; they split the agent's actions
+constexpr int kSimAgents = kSimBlock / kSimLanes;  // agents per workgroup
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
@@ -106,15 +108,38 @@ ev_kernel(const LcmEvDesc* __restrict__ descs, int max_rows) {
 // ---------------------------------------------------------------------------
 // Partition agents by regime (order within a list does not affect results).
 // ---------------------------------------------------------------------------
-__global__ void partition_kernel(const int32_t* __restrict__ regime, const int32_t* __restrict__ start,
-                                 int n, int period, int n_regimes, int32_t* __restrict__ counts,
-                                 int32_t* __restrict__ order) {
+// Counts per regime in LDS first, then one global atomic per (workgroup,
+// regime) reserves the workgroup's range: contention on the global counters
+// stays at one update per workgroup instead of one per agent. The same pass
+// carries every agent's state into the next panel (agents not simulated
+// this period keep it; simulated ones are overwritten by the simulation
+// kernel) and marks the period's actions as not simulated (-1).
+template <typename T>
+__global__ void __launch_bounds__(256)
+partition_kernel(LcmAgents cur, LcmAgents nxt, int32_t* __restrict__ action_out,
+                 int n, int period, int n_regimes, int32_t* __restrict__ counts,
+                 int32_t* __restrict__ order) {
+  __shared__ int lcount[LCM_SIM_MAX_REGIMES];
+  __shared__ int lbase[LCM_SIM_MAX_REGIMES];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int r = regime[i];
-  if (start[i] > period || r < 0 || r >= n_regimes) return;
-  const int slot = atomicAdd(&counts[r], 1);
-  order[int64_t(r) * n + slot] = i;
+  int r = -1;
+  if (i < n) {
+    r = cur.regime[i];
+    nxt.regime[i] = r;
+    nxt.sdd[i] = cur.sdd[i];
+    static_cast<T*>(nxt.w)[i] = static_cast<const T*>(cur.w)[i];
+    static_cast<T*>(nxt.z)[i] = static_cast<const T*>(cur.z)[i];
+    action_out[i] = -1;
+    if (cur.start[i] > period || r < 0 || r >= n_regimes) r = -1;
+  }
+  if (threadIdx.x < LCM_SIM_MAX_REGIMES) lcount[threadIdx.x] = 0;
+  __syncthreads();
+  const int local = r >= 0 ? atomicAdd(&lcount[r], 1) : 0;
+  __syncthreads();
+  if (threadIdx.x < n_regimes && lcount[threadIdx.x] > 0)
+    lbase[threadIdx.x] = atomicAdd(&counts[threadIdx.x], lcount[threadIdx.x]);
+  __syncthreads();
+  if (r >= 0) order[int64_t(r) * n + lbase[r] + local] = i;
 }
 
 __global__ void clear_kernel(int32_t* p, int n) {
@@ -122,10 +147,6 @@ __global__ void clear_kernel(int32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 
-__global__ void fill_kernel(int32_t* p, int n, int32_t v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
 
 // ---------------------------------------------------------------------------
 // Per-agent argmax over actions and next-state draw.
@@ -152,13 +173,33 @@ __device__ __forceinline__ T sim_coordinate(int kind, int n, T start, T stop, co
   return T(dn) + (x - pts[dn]) / (pts[up] - pts[dn]);
 }
 
-template <typename T>
+template <typename T, bool FINITE>
 __device__ __forceinline__ T interp_at(const T* row, int n, T coord) {
   if (n < 2) return row[0];
   T fl = fmin(fmax(floor(coord), T(0)), T(n - 2));
   const int lo = int(fl);
   const T t = coord - fl;
+  if constexpr (FINITE) return (T(1) - t) * row[lo] + t * row[lo + 1];
   return zero_safe(T(1) - t, row[lo]) + zero_safe(t, row[lo + 1]);
+}
+
+// Running first-argmax of one lane and its merge with another lane's: the
+// larger value wins, equal values keep the smaller flat action index.
+template <typename T>
+struct SimBest {
+  T best;
+  int idx;
+  bool nan;
+};
+
+template <typename T>
+__device__ __forceinline__ SimBest<T> sim_merge(SimBest<T> a, SimBest<T> b) {
+  a.nan |= b.nan;
+  if (b.idx >= 0 && (a.idx < 0 || b.best > a.best || (b.best == a.best && b.idx < a.idx))) {
+    a.best = b.best;
+    a.idx = b.idx;
+  }
+  return a;
 }
 
 template <typename T>
@@ -197,8 +238,10 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   const LcmSimDesc* __restrict__ D = descs + r;
   if (D->U0 == nullptr) return;
   const int cnt = counts[r];
-  const int slot = blockIdx.x * kSimBlock + threadIdx.x;
-  if (int(blockIdx.x) * kSimBlock >= cnt) return;  // whole block idle
+  // kSimLanes consecutive lanes serve one agent and split its actions
+  const int lane = threadIdx.x % kSimLanes;
+  const int slot = blockIdx.x * kSimAgents + threadIdx.x / kSimLanes;
+  if (int(blockIdx.x) * kSimAgents >= cnt) return;  // whole block idle
   const bool valid = slot < cnt;
   const int agent = valid ? order[int64_t(r) * n_agents + slot] : 0;
 
@@ -255,19 +298,26 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   }
   const T g_start = T(D->g_start), g_stop = T(D->g_stop);
   const T beta = T(D->beta);
+  const bool lin = interp && D->g_kind == LCM_GRID_LIN && g_n >= 2;
+  const T g_step = lin ? (g_stop - g_start) / T(g_n - 1) : T(1);
+  const T g_inv = T(1) / g_step;
 
   T best = -INFINITY;
   int bidx = -1;
   bool nanf = false;
   for (int d = 0; d < n_d; ++d) {
     __syncthreads();
+    int nonfinite = 0;
     if (interp) {
       const T* EVg = static_cast<const T*>(D->EV);
       for (int i = threadIdx.x; i < n_rows * g_n; i += kSimBlock) {
         const int rr = i / g_n, k = i % g_n;
-        ev[i] = EVg[(int64_t(rr) * n_d + d) * g_n + k];
+        const T v = EVg[(int64_t(rr) * n_d + d) * g_n + k];
+        ev[i] = v;
+        nonfinite |= !is_finite(v);
       }
     }
+    const bool finite = __syncthreads_or(nonfinite) == 0;
     T cs = T(0);
     if (!D->terminal) {
       const T* CS = static_cast<const T*>(D->CS);
@@ -297,7 +347,7 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
       }
       __syncthreads();
       const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
-      for (int i = 0; i < cn; ++i) {
+      for (int i = lane; i < cn; i += kSimLanes) {
         T lo = mine[3 * kSimChunk + i], hi = mine[4 * kSimChunk + i];
         if (lo_feat) {
           T l = mine[5 * kSimChunk + i];
@@ -321,10 +371,22 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
         } else if (interp) {
           T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
           for (int k = 0; k < nf; ++k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
-          const T coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
-          const T v_lo = interp_at(ev + int64_t(row_lo) * g_n, g_n, coord);
-          T v = zero_safe(T(1) - tz, v_lo);
-          if (n_nodes > 1) v += zero_safe(tz, interp_at(ev + int64_t(row_hi) * g_n, g_n, coord));
+          T coord;
+          if (lin) {
+            // linear grid: the same coordinate as the grid-search kernel
+            if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
+            else coord = (x - g_start) * g_inv;
+          } else {
+            coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
+          }
+          T v;
+          if (finite) {
+            v = (T(1) - tz) * interp_at<T, true>(ev + int64_t(row_lo) * g_n, g_n, coord);
+            if (n_nodes > 1) v += tz * interp_at<T, true>(ev + int64_t(row_hi) * g_n, g_n, coord);
+          } else {
+            v = zero_safe(T(1) - tz, interp_at<T, false>(ev + int64_t(row_lo) * g_n, g_n, coord));
+            if (n_nodes > 1) v += zero_safe(tz, interp_at<T, false>(ev + int64_t(row_hi) * g_n, g_n, coord));
+          }
           q = u + beta * (cs + v);
         } else {
           q = u + beta * cs;
@@ -339,7 +401,20 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
       }
     }
   }
-  if (!valid) return;
+  // merge the agent's lanes (butterfly within the aligned group of kSimLanes)
+  SimBest<T> mb{best, bidx, nanf};
+#pragma unroll
+  for (int off = kSimLanes / 2; off >= 1; off >>= 1) {
+    SimBest<T> o;
+    o.best = __shfl_xor(mb.best, off);
+    o.idx = __shfl_xor(mb.idx, off);
+    o.nan = __shfl_xor(int(mb.nan), off) != 0;
+    mb = sim_merge(mb, o);
+  }
+  best = mb.best;
+  bidx = mb.idx;
+  nanf = mb.nan;
+  if (!valid || lane != 0) return;
 
   // record this period
   const int64_t o = int64_t(period) * n_agents + agent;
@@ -442,19 +517,17 @@ extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_si
     return fail(LCM_E_ARG, "lcm_sim_period: bad sizes");
   if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_sim_period: LDS request above 160 KiB");
   const int nb = (n_agents + 255) / 256;
+  const int nb_sim = (n_agents + kSimAgents - 1) / kSimAgents;
   hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, ctx->stream, d_counts, n_regimes);
-  hipLaunchKernelGGL(partition_kernel, dim3(nb), dim3(256), 0, ctx->stream, cur->regime, cur->start,
-                     n_agents, period, n_regimes, d_counts, d_order);
-  // agents not simulated this period keep their state; the next-period regime
-  // starts as the current one and is overwritten for simulated agents
-  LCM_HIP(hipMemcpyAsync(next->regime, cur->regime, sizeof(int32_t) * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
-  LCM_HIP(hipMemcpyAsync(next->sdd, cur->sdd, sizeof(int32_t) * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
-  LCM_HIP(hipMemcpyAsync(next->w, cur->w, sz * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
-  LCM_HIP(hipMemcpyAsync(next->z, cur->z, sz * n_agents, hipMemcpyDeviceToDevice, ctx->stream));
-  hipLaunchKernelGGL(fill_kernel, dim3(nb), dim3(256), 0, ctx->stream,
-                     out->action + int64_t(period) * n_agents, n_agents, -1);
-  dim3 grid(nb, n_regimes);
+  int32_t* action_out = out->action + int64_t(period) * n_agents;
+  if (sz == 8)
+    hipLaunchKernelGGL(partition_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, *cur, *next, action_out,
+                       n_agents, period, n_regimes, d_counts, d_order);
+  else
+    hipLaunchKernelGGL(partition_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, *cur, *next, action_out,
+                       n_agents, period, n_regimes, d_counts, d_order);
+  dim3 grid(nb_sim, n_regimes);
   const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
   if (dtype == LCM_DTYPE_F32) {
     if (lds_bytes > 64 * 1024)
