@@ -257,31 +257,49 @@ def simulation_leg(args):
     return out
 
 
-def serving_leg(args):
-    """Poisson arrivals (rate per second) for a fixed window into the
-    continuous-batching server; requests mix the deterministic and the AR(1)
-    income retirement models, horizons U{20..60}, grids U{200..4000}. The
-    same request sequence solved one at a time gives the plain baseline."""
+def _serve_window(args, rate, seconds, seed):
     from pylcm_amd import examples as ex
-    from pylcm_amd._gs_program import GsProgram
     from pylcm_amd.serving import SolveServer, serve_poisson
 
-    dt = np.float32
-    server = SolveServer(dtype=dt, tile=args.serve_tile, fetch="first", lower_threads=2)
+    server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=4,
+                         lower_procs=args.serve_procs)
     make = ex.serving_request_factory(seed=7)
     t0 = time.perf_counter()
-    reqs = serve_poisson(server, make, rate=args.serve_rate, duration=args.serve_seconds, seed=11)
+    reqs = serve_poisson(server, make, rate=rate, duration=seconds, seed=seed)
     wall = time.perf_counter() - t0
     server.close()
     ok = [r for r in reqs if r.error is None]
     lat = np.array([r.latency for r in ok])
     span = max(r.t_done for r in ok) - min(r.t_submit for r in ok)
-    log(f"serving: {len(ok)}/{len(reqs)} requests in {wall:.1f}s, {len(ok)/span:.1f} solves/s, "
-        f"p50 {1e3*np.percentile(lat, 50):.1f} ms, p99 {1e3*np.percentile(lat, 99):.1f} ms, "
-        f"{server.launches} fused launches")
+    out = {"arrival_rate": rate, "seconds": seconds, "requests": len(reqs), "completed": len(ok),
+           "solves_per_s": round(len(ok) / span, 2),
+           "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+           "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+           "fused_launches": server.launches,
+           "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
+           "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
+           "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1)}
+    log(f"serving at {rate:g}/s: {len(ok)}/{len(reqs)} requests in {wall:.1f}s, {out['solves_per_s']:.1f} solves/s, "
+        f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {server.launches} fused launches")
+    return out
+
+
+def serving_leg(args):
+    """Poisson arrivals into the continuous-batching server (requests lowered
+    in worker processes); requests mix the deterministic and the AR(1) income
+    retirement models, horizons U{20..60}, grids U{200..4000}. One window at
+    the stated rate (latency), one well above capacity (saturation
+    throughput). The same request sequence solved one at a time gives the
+    plain baseline."""
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import GsProgram
+
+    dt = np.float32
+    win = _serve_window(args, args.serve_rate, args.serve_seconds, seed=11)
+    sat = _serve_window(args, args.serve_sat_rate, args.serve_sat_seconds, seed=12) if args.serve_sat_seconds > 0 else None
     # plain: the same sequence, one request at a time (lower, place, solve, fetch period 0)
     make2 = ex.serving_request_factory(seed=7)
-    n_plain = min(args.serve_plain_sample, len(reqs))
+    n_plain = min(args.serve_plain_sample, win["requests"])
     plain = []
     for i in range(n_plain):
         m, p = make2(i)
@@ -298,22 +316,19 @@ def serving_leg(args):
         plain.append(time.perf_counter() - t_a)
         del pr
     plain = np.array(plain)
+    plain_sps = 1.0 / plain.mean()
     return {
         "workload": f"Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g}s; deterministic and AR(1)-income "
-                    "retirement models, T U{20..60}, grids U{200..4000}, float32, period-0 V returned",
-        "requests": len(reqs), "completed": len(ok),
-        "solves_per_s": round(len(ok) / span, 2),
-        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
-        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
-        "fused_launches": server.launches,
-        "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
-        "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
-        "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
-        "plain_serial": {"solves_per_s": round(1.0 / plain.mean(), 2),
+                    "retirement models, T U{20..60}, grids U{200..4000}, float32, period-0 V returned; "
+                    f"requests lowered in {args.serve_procs} worker processes",
+        **win,
+        "saturation": sat,
+        "plain_serial": {"solves_per_s": round(plain_sps, 2),
                          "p50_ms": round(1e3 * float(np.percentile(plain, 50)), 2),
                          "p99_ms": round(1e3 * float(np.percentile(plain, 99)), 2),
                          "sample": n_plain},
-        "serving_vs_plain": round((len(ok) / span) * plain.mean(), 2),
+        "serving_vs_plain": round(win["solves_per_s"] / plain_sps, 2),
+        "saturation_vs_plain": round(sat["solves_per_s"] / plain_sps, 2) if sat else None,
     }
 
 
@@ -336,6 +351,9 @@ def main():
     ap.add_argument("--sim-cpu-agents", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-sat-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-sat-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-procs", type=int, default=8)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
     ap.add_argument("--serve-tile", type=int, default=256)
     args = ap.parse_args()

@@ -177,6 +177,13 @@ def lower_problem(model, params, dtype):
     return tabs, rg
 
 
+def step_lds_bytes(dtype, gmax, kmax):
+    """`lcm_gs_lds_bytes` for each step's largest next grid and constraint count."""
+    code = nat.LCM_DTYPE_F64 if np.dtype(dtype) == np.float64 else nat.LCM_DTYPE_F32
+    L = nat.load_library()
+    return [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
+
+
 class GsProgram:
     """Packed, device-resident grid-search solves.
 
@@ -256,6 +263,7 @@ class GsProgram:
         self.flops_by_step = [0.0] * S
         self.bytes_by_step = [0.0] * S
         dcache = {}
+        task_grids = {}
         for p, (model, tabs, rg) in enumerate(zip(self.models, self.problems, self._rg)):
             order = sorted(tabs.items(), key=lambda kv: (-kv[0][0], model.regime_code(kv[0][1])))
             for (t, r), pt in order:
@@ -275,33 +283,43 @@ class GsProgram:
                 # algorithmic HBM bytes: next V rows read, V and policy written
                 n_next = sum(nj * d["g_n"] for nj in d["nj"])
                 self.bytes_by_step[s] += isz * (n_next + n_states) + 4 * n_states
-                for sd in range(lay.n_sd):
-                    for w0 in range(0, lay.n_w, self.tile):
-                        tasks[s].append((di, sd, w0))
+                # tasks (descriptor, discrete state, first wealth node), sd-major
+                key = (lay.n_sd, lay.n_w)
+                grid = task_grids.get(key)
+                if grid is None:
+                    sd_i, w_i = np.meshgrid(np.arange(lay.n_sd, dtype=np.int32),
+                                            np.arange(0, lay.n_w, self.tile, dtype=np.int32), indexing="ij")
+                    grid = task_grids[key] = np.stack([sd_i.ravel(), w_i.ravel()], axis=1)
+                tasks[s].append((di, grid))
         self.n_desc = len(descs)
         self.desc_off = ar.reserve(self.n_desc * DESC_DTYPE.itemsize)
         self.task_off, self.n_tasks = [], []
+        self.tasks_host = []
         for s in range(S):
-            tl = tasks[s]
-            if not tl:
+            parts = tasks[s]
+            n = sum(len(g) for _, g in parts)
+            tl_a = np.empty((n, 3), dtype=np.int32)
+            i = 0
+            for di, g in parts:
+                tl_a[i: i + len(g), 0] = di
+                tl_a[i: i + len(g), 1:] = g
+                i += len(g)
+            self.tasks_host.append(tl_a)
+            if not n:
                 self.task_off.append(0)
                 self.n_tasks.append(0)
                 continue
-            n = len(tl)
             npad = (n + 7) // 8 * 8
             arr = np.zeros(npad, dtype=TASK_DTYPE)
-            tl_a = np.asarray(tl, dtype=np.int32)
             arr["desc"][:n] = tl_a[:, 0]
             arr["sd"][:n] = tl_a[:, 1]
             arr["w0"][:n] = tl_a[:, 2]
             arr["desc"][n:] = -1
             self.task_off.append(ar.add(arr, dedupe=False))
             self.n_tasks.append(npad)
-        self.tasks_host = [np.asarray(tl, dtype=np.int32).reshape(-1, 3) for tl in tasks]
         self.wdep_by_step = wdep
-        code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
-        L = nat.load_library()
-        self.lds_by_step = [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
+        self._lds_g, self._lds_k = gmax, kmax
+        self._lds_by_step = None
         self.total_bytes = _align(ar.size)
         self._arena, self._descs = ar, descs
         self.out_host = np.zeros(self.out_end, dtype=np.uint8)
@@ -309,6 +327,14 @@ class GsProgram:
         if not self._defer:
             self.buf = nat.DeviceBuffer(self.total_bytes)
             self.materialize(self.buf.ptr)
+
+    @property
+    def lds_by_step(self):
+        """Dynamic LDS bytes of each step's launch (asked of the library on
+        first use, so packing itself loads no native code)."""
+        if self._lds_by_step is None:
+            self._lds_by_step = step_lds_bytes(self.dt, self._lds_g, self._lds_k)
+        return self._lds_by_step
 
     def materialize(self, ptr: int, upload_ctx=None):
         """Write the descriptors for device address `ptr` and upload the image

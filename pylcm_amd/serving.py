@@ -29,9 +29,45 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from pylcm_amd import _native as nat
-from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_problem
+from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_problem, step_lds_bytes
 
 ALIGN = 256
+
+
+def _lower_in_worker(blob: bytes, dtype: str, tile: int) -> dict:
+    """Lower and pack one request in a worker process (no device code is
+    loaded there): everything the server needs to place and run it."""
+    import cloudpickle
+
+    model, params = cloudpickle.loads(blob)
+    dt = np.dtype(dtype)
+    low = lower_problem(model, params, dt)
+    prog = GsProgram.of([(model, params)], dt, lowered=[low], tile=tile, defer=True)
+    D0, masks = prog.relocatable_records()
+    return {"image": prog._arena.image(prog.out_start), "D0": D0, "masks": masks,
+            "tasks_host": prog.tasks_host, "outs": prog.outs, "n_steps": prog.n_steps,
+            "lds_g": prog._lds_g, "lds_k": prog._lds_k, "wdep_by_step": prog.wdep_by_step,
+            "total_bytes": prog.total_bytes, "out_start": prog.out_start, "n_desc": prog.n_desc}
+
+
+def _warm_worker() -> int:
+    """Import what `_lower_in_worker` needs, so the first request pays no start-up."""
+    import cloudpickle  # noqa: F401
+
+    from pylcm_amd import examples  # noqa: F401
+
+    return 0
+
+
+class _PackedRequest:
+    """The parts of a `GsProgram` the server uses, rebuilt from a worker's result."""
+
+    def __init__(self, data: dict, dtype):
+        self.dt = np.dtype(dtype)
+        for k in ("tasks_host", "outs", "n_steps", "wdep_by_step", "total_bytes", "out_start", "n_desc"):
+            setattr(self, k, data[k])
+        self.lds_by_step = step_lds_bytes(self.dt, data["lds_g"], data["lds_k"])
+        self.base = None
 
 
 class _Slab:
@@ -85,6 +121,7 @@ class Request:
     desc: tuple | None = None
     tasks: list | None = None
     done_event: object = None
+    future: object = None
     result: dict | None = None
     done: threading.Event = field(default_factory=threading.Event)
 
@@ -99,11 +136,15 @@ class SolveServer:
     `budget_bytes` caps the device memory the server uses (default: the given
     fraction of the free HBM at start). `tile` is shared by all requests (the
     fused launch runs one tile shape). `max_inflight` caps admitted requests.
+    `lower_procs` > 0 lowers requests in that many spawned worker processes
+    (models are sent with cloudpickle); as with any spawned pool, the main
+    script must guard its entry point with `if __name__ == "__main__":`.
     """
 
     def __init__(self, *, dtype=np.float32, tile: int = 256, budget_bytes: int | None = None,
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
-                 max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1):
+                 max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1,
+                 lower_procs: int = 0):
         self.dt = np.dtype(dtype)
         self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         self.tile = int(tile)
@@ -143,6 +184,18 @@ class SolveServer:
         self.timing = {"admit": 0.0, "launch": 0.0, "retire": 0.0}
         self._launch_events = []  # (start, end) per fused launch, for the GPU busy time
         self._stop = False
+        # lower_procs > 0: lowering runs in that many worker processes (the
+        # model travels by cloudpickle) and the threads only place requests
+        self._pool = None
+        if lower_procs > 0:
+            import multiprocessing
+            from concurrent.futures import ProcessPoolExecutor
+
+            self._pool = ProcessPoolExecutor(max_workers=int(lower_procs),
+                                             mp_context=multiprocessing.get_context("spawn"))
+            for f in [self._pool.submit(_warm_worker) for _ in range(int(lower_procs))]:
+                f.result()
+            lower_threads = max(lower_threads, 2)
         self._workers = [threading.Thread(target=self._lower_worker, daemon=True) for _ in range(lower_threads)]
         for w in self._workers:
             w.start()
@@ -151,6 +204,11 @@ class SolveServer:
     def submit(self, model, params, t_submit: float | None = None) -> Request:
         req = Request(self._next_id, model, params, t_submit if t_submit is not None else time.perf_counter())
         self._next_id += 1
+        if self._pool is not None:
+            import cloudpickle
+
+            req.future = self._pool.submit(_lower_in_worker, cloudpickle.dumps((model, params)),
+                                           self.dt.str, self.tile)
         self._lowq.put(req)
         return req
 
@@ -160,6 +218,8 @@ class SolveServer:
             self._lowq.put(None)
         with self._mem_cv:
             self._mem_cv.notify_all()
+        if self._pool is not None:
+            self._pool.shutdown(wait=False, cancel_futures=True)
 
     # -- host lowering and admission (worker threads) ------------------------------
     def _lower_worker(self):
@@ -169,11 +229,17 @@ class SolveServer:
             if req is None:
                 return
             try:
-                low = lower_problem(req.model, req.params, self.dt)
-                prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
-                                    ctx=self.ctx, defer=True)
-                D0, masks = prog.relocatable_records()
-                image = prog._arena.image(prog.out_start)
+                if req.future is not None:
+                    data = req.future.result()
+                    req.future = None
+                    prog = _PackedRequest(data, self.dt)
+                    D0, masks, image = data["D0"], data["masks"], data["image"]
+                else:
+                    low = lower_problem(req.model, req.params, self.dt)
+                    prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
+                                        ctx=self.ctx, defer=True)
+                    D0, masks = prog.relocatable_records()
+                    image = prog._arena.image(prog.out_start)
                 req.prog = prog
                 req.t_lowered = time.perf_counter()
                 self._place(req, copy_ctx, image, D0, masks)

@@ -82,3 +82,21 @@ def test_admission_by_memory_budget(native):
         ref = _alone(m, p, 128)
         for r in ref[0]:
             assert np.array_equal(h.result[0][r], ref[0][r])
+
+
+def test_worker_process_lowering_same_results(native):
+    """Requests lowered in spawned worker processes give the same value
+    functions, bit for bit, as each request solved alone."""
+    from pylcm_amd.serving import SolveServer
+
+    reqs = _requests(8, seed=5, max_grid=300)
+    server = SolveServer(tile=128, fetch="all", budget_bytes=1 << 30, lower_procs=2)
+    handles = [server.submit(m, p) for m, p in reqs]
+    server.drain()
+    server.close()
+    for h, (m, p) in zip(handles, reqs):
+        assert h.error is None, h.error
+        ref = _alone(m, p, 128)
+        for t in ref:
+            for r in ref[t]:
+                assert np.array_equal(h.result[t][r], ref[t][r]), (h.rid, t, r)

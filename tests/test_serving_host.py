@@ -69,3 +69,36 @@ def test_fused_tasks_address_each_requests_own_descriptors():
             assert np.all((r["V_out"] >= lo) & (r["V_out"] < hi))
             assert np.all((r["U"] >= lo) & (r["U"] < hi))
             assert np.all(tasks[:, 1] < r["n_sd"]) and np.all(tasks[:, 2] < r["n_w"])
+
+
+def test_worker_process_lowering_matches_in_process():
+    """Lowering in a spawned worker (the model travels by cloudpickle) gives
+    the same device image, relocatable descriptor records and task lists as
+    lowering in the server's own process."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+
+    import cloudpickle
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import GsProgram, lower_problem
+    from pylcm_amd.serving import _lower_in_worker
+
+    make = ex.serving_request_factory(seed=3, horizons=(4, 6), grids=(40, 300))
+    reqs = [make(i) for i in range(3)]
+    with ProcessPoolExecutor(max_workers=2, mp_context=multiprocessing.get_context("spawn")) as pool:
+        outs = list(pool.map(_lower_in_worker, [cloudpickle.dumps(r) for r in reqs],
+                             [np.dtype(np.float32).str] * 3, [128] * 3))
+    for (m, p), got in zip(reqs, outs):
+        prog = GsProgram.of([(m, p)], np.float32, lowered=[lower_problem(m, p, np.float32)], tile=128, defer=True)
+        D0, masks = prog.relocatable_records()
+        assert got["image"].tobytes() == prog._arena.image(prog.out_start).tobytes()
+        assert got["D0"].tobytes() == D0.tobytes()
+        assert sorted(got["masks"]) == sorted(masks)
+        assert all(np.array_equal(got["masks"][f], masks[f]) for f in masks)
+        assert len(got["tasks_host"]) == len(prog.tasks_host)
+        assert all(np.array_equal(a, b) for a, b in zip(got["tasks_host"], prog.tasks_host))
+        assert (got["total_bytes"], got["out_start"], got["n_desc"], got["n_steps"]) == (
+            prog.total_bytes, prog.out_start, prog.n_desc, prog.n_steps)
+        assert got["outs"][0].keys() == prog.outs[0].keys()
