@@ -118,7 +118,36 @@ def dcegm_leg(args):
         "host_build_s": round(build_s, 3),
         "vs_grid_search_4000": {"median_rel_diff": float(np.median(d)), "p99_rel_diff": float(np.percentile(d, 99)),
                                 "note": "grid resolution; the two lowest wealth nodes excluded"},
+        "columns_per_s": round(n_cols / (ms_solve * 1e-3), 1),
     }
+    if args.dcegm_batch > 0:
+        from pylcm_amd._egm_program import EgmBatch
+
+        rng = np.random.default_rng(20261203)
+        plist = [ex.dcegm_params(T, discount_factor=float(rng.uniform(0.93, 0.98)), crra=float(rng.uniform(1.5, 3.0)),
+                                 sigma=float(rng.uniform(0.1, 0.2))) for _ in range(args.dcegm_batch)]
+        plist[0] = p
+        t0 = time.perf_counter()
+        batch = EgmBatch(m, plist, dt, ctx=prog.ctx)
+        bbuild = time.perf_counter() - t0
+        batch.run()
+        breps = 3
+        t0 = time.perf_counter()
+        for _ in range(breps):
+            batch.run(sync=False)
+        prog.ctx.sync()
+        ms_batch = 1e3 * (time.perf_counter() - t0) / breps
+        Vb = batch.value_functions(0)
+        same = all(np.array_equal(Vb[t][r], V[t][r]) for t in V for r in V[t])
+        out["batch"] = {"draws": args.dcegm_batch, "ms_per_batch": round(ms_batch, 3),
+                        "solves_per_s": round(args.dcegm_batch / (ms_batch * 1e-3), 2),
+                        "columns_per_launch": sum(pr._period[max(pr._period)][1] for pr in batch.progs),
+                        "vs_single": round((args.dcegm_batch / ms_batch) * ms_solve, 2),
+                        "host_build_s": round(bbuild, 2), "device_MiB": round(batch.total_bytes / 2**20, 1),
+                        "draw0_equals_single_solve": bool(same)}
+        log(f"dcegm batch: {args.dcegm_batch} draws in {ms_batch:.2f} ms -> {out['batch']['solves_per_s']:.1f} solves/s "
+            f"({out['batch']['vs_single']}x single); draw 0 equals the single solve: {same}")
+        del batch
     if not args.no_cpu:
         from oracle import dcegm as od
         from tests.parity import rel_err
@@ -346,6 +375,7 @@ def main():
     ap.add_argument("--copy-calib-mib", type=int, default=256)
     ap.add_argument("--no-dcegm", action="store_true")
     ap.add_argument("--dcegm-periods", type=int, default=50)
+    ap.add_argument("--dcegm-batch", type=int, default=64)
     ap.add_argument("--no-sim", action="store_true")
     ap.add_argument("--sim-agents", type=int, default=100_000)
     ap.add_argument("--sim-cpu-agents", type=int, default=500)

@@ -311,7 +311,9 @@ class EgmProgram:
         self.max_cand = max(self.max_cand, n_cand)
 
     # ------------------------------------------------------------------ packing
-    def _pack(self):
+    def _reserve(self):
+        """Reserve the child row tables, the candidate scratch and the
+        per-period descriptor arrays in the arena (no device work)."""
         ar = self.ar
         # child row tables: one LcmEgmRow array per (t, regime) that has rows
         self._rowtab = {}
@@ -319,16 +321,20 @@ class EgmProgram:
             if info["kind"] == "rows":
                 self._rowtab[key] = ar.reserve(len(info["rows"]) * ROW.itemsize)
         self._period = {}
-        cand_per_col = ((4 * self.max_cand + 1) * 8 + 255) // 256 * 256
+        self._cand_per_col = ((4 * self.max_cand + 1) * 8 + 255) // 256 * 256
         max_cols = max((sum(s["n_cols"] for s in specs) for specs in self._col_specs.values()), default=0)
-        self.cand_off = ar.reserve(max(max_cols, 1) * max(cand_per_col, 8))
+        self.cand_off = ar.reserve(max(max_cols, 1) * max(self._cand_per_col, 8))
         for t, specs in self._col_specs.items():
             n_cols = sum(s["n_cols"] for s in specs)
             aggs = sum(self.col_out[(tt, r)]["n_sd"] for tt, r, _ in self._agg_specs[t])
             self._period[t] = (ar.reserve(n_cols * COL.itemsize), n_cols, ar.reserve(aggs * AGG.itemsize), aggs)
         self.total_bytes = ar.size
-        self.buf = nat.DeviceBuffer(max(ar.size, 256))
-        base = self.buf.ptr
+
+    def _image(self, base):
+        """The arena image for device address `base` (row tables and
+        descriptors written), with {t: (columns, aggregates)} ctypes arrays
+        and {t: launch sizes}."""
+        ar = self.ar
         img = ar.image()
         if img.size < ar.size:
             img = np.concatenate([img, np.zeros(ar.size - img.size, np.uint8)])
@@ -343,7 +349,8 @@ class EgmProgram:
             for i, (g, v, m, nv, ln) in enumerate(rows):
                 arr[i] = nat.LcmEgmRow(base + g, base + v, base + m, base + nv, ln, 0)
             put(off, arr)
-        self.lds_by_t = {}
+        descs, sizes = {}, {}
+        cand_per_col = self._cand_per_col
         for t, specs in self._col_specs.items():
             col_off, n_cols, agg_off, n_aggs = self._period[t]
             arr = (nat.LcmEgmCol * n_cols)()
@@ -396,16 +403,24 @@ class EgmProgram:
                     ai += 1
                 max_w = max(max_w, o["n_w"])
             put(agg_off, aggs)
-            self.lds_by_t[t] = (int(nat.lib().lcm_egm_lds_bytes(max_cand)), max_w,
-                                max(sp["n_rows"] for sp in specs), max(sp["n_s"] for sp in specs))
-        nat.h2d(self.ctx, self.buf.ptr, img[: ar.size])
+            descs[t] = (arr, aggs)
+            sizes[t] = (int(nat.lib().lcm_egm_lds_bytes(max_cand)), max_w,
+                        max(sp["n_rows"] for sp in specs), max(sp["n_s"] for sp in specs))
+        return img, descs, sizes
+
+    def _pack(self):
+        self._reserve()
+        self.buf = nat.DeviceBuffer(max(self.ar.size, 256))
+        self.base = self.buf.ptr
+        img, _, self.lds_by_t = self._image(self.base)
+        nat.h2d(self.ctx, self.base, img[: self.ar.size])
         self.ctx.sync()
 
     # ------------------------------------------------------------------ running
     def run(self, sync=True):
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
-        base = self.buf.ptr
+        base = self.base
         for t in sorted(self._period, reverse=True):
             col_off, n_cols, agg_off, n_aggs = self._period[t]
             lds, max_w, max_rows, max_s = self.lds_by_t[t]
@@ -418,7 +433,7 @@ class EgmProgram:
 
     def _fetch(self, off, nbytes, dtype):
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
-        nat.d2h(self.ctx, self.buf.ptr + off, out)
+        nat.d2h(self.ctx, self.base + off, out)
         return out
 
     def value_functions(self):
@@ -464,3 +479,79 @@ def solve_dcegm_model(model, params, dtype):
     prog = EgmProgram(model, params, dtype)
     prog.run()
     return prog.value_functions(), prog.policies()
+
+
+class EgmBatch:
+    """Many parameter sets of one DC-EGM model: per period ONE column launch
+    and ONE aggregate launch hold the columns of every set. Each set keeps
+    its own arena (tables, carry rows, scratch) inside one device buffer, so
+    its results are those of its own `EgmProgram`."""
+
+    def __init__(self, model, params_list, dtype, ctx=None):
+        _abi_check()
+        self.dt = np.dtype(dtype)
+        self.ctx = ctx or nat.Context()
+        self.progs = [EgmProgram(model, p, dtype, device=False) for p in params_list]
+        offs, total = [], 0
+        for pr in self.progs:
+            pr._reserve()
+            offs.append(total)
+            total += (pr.ar.size + 255) // 256 * 256
+        periods = sorted(self.progs[0]._period) if self.progs else []
+        # the fused per-period descriptor arrays follow the programs' arenas
+        self._fused = {}
+        for t in periods:
+            n_cols = sum(pr._period[t][1] for pr in self.progs)
+            n_aggs = sum(pr._period[t][3] for pr in self.progs)
+            col_off = total
+            total += (n_cols * COL.itemsize + 255) // 256 * 256
+            agg_off = total
+            total += (n_aggs * AGG.itemsize + 255) // 256 * 256
+            self._fused[t] = (col_off, n_cols, agg_off, n_aggs)
+        self.total_bytes = total
+        self.buf = nat.DeviceBuffer(max(total, 256))
+        base = self.buf.ptr
+        cols = {t: [] for t in periods}
+        aggs = {t: [] for t in periods}
+        self.lds_by_t = {}
+        for pr, off in zip(self.progs, offs):
+            pr.ctx = self.ctx
+            pr.buf = None
+            pr.base = base + off
+            img, descs, sizes = pr._image(pr.base)
+            nat.h2d(self.ctx, pr.base, img[: pr.ar.size])
+            for t in periods:
+                cols[t].append(bytes(descs[t][0]))
+                aggs[t].append(bytes(descs[t][1]))
+                lds, mw, mr, ms = sizes[t]
+                o = self.lds_by_t.get(t, (0, 0, 0, 0))
+                self.lds_by_t[t] = (max(o[0], lds), max(o[1], mw), max(o[2], mr), max(o[3], ms))
+        for t in periods:
+            col_off, _, agg_off, _ = self._fused[t]
+            c = np.frombuffer(b"".join(cols[t]), dtype=np.uint8)
+            a = np.frombuffer(b"".join(aggs[t]), dtype=np.uint8)
+            if c.size:
+                nat.h2d(self.ctx, base + col_off, c)
+            if a.size:
+                nat.h2d(self.ctx, base + agg_off, a)
+        self.ctx.sync()
+
+    def run(self, sync=True):
+        L = nat.lib()
+        code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
+        base = self.buf.ptr
+        for t in sorted(self._fused, reverse=True):
+            col_off, n_cols, agg_off, n_aggs = self._fused[t]
+            lds, max_w, max_rows, max_s = self.lds_by_t[t]
+            nat._check(L.lcm_egm_launch(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
+                                        max_rows, max_s), f"lcm_egm_launch (period {t})")
+            nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
+                       f"lcm_egm_aggregate (period {t})")
+        if sync:
+            self.ctx.sync()
+
+    def value_functions(self, i):
+        return self.progs[i].value_functions()
+
+    def policies(self, i):
+        return self.progs[i].policies()

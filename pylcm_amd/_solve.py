@@ -5,7 +5,6 @@ from __future__ import annotations
 import numpy as np
 
 from pylcm_amd._gs_program import GsProgram, lower_problem
-from pylcm_amd.exceptions import UnsupportedModelError
 
 
 def _split(results):
@@ -34,7 +33,15 @@ def solve_model(model, params, dtype):
 
 def solve_batch(model, params_list, dtype, *, tile=None, use_graph=True, return_policy=False):
     if _has_dcegm(model):
-        raise UnsupportedModelError("batched solves of DC-EGM regimes are not implemented")
+        from pylcm_amd._egm_program import EgmBatch
+
+        batch = EgmBatch(model, params_list, dtype)
+        batch.run()
+        out = []
+        for i in range(len(params_list)):
+            V = batch.value_functions(i)
+            out.append((V, batch.policies(i)) if return_policy else V)
+        return out
     prog = GsProgram(model, params_list, dtype, tile=tile)
     prog.run(use_graph=use_graph)
     prog.fetch()

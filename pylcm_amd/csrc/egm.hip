@@ -391,9 +391,14 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double* vals = reinterpret_cast<double*>(lds_raw);  // [Q][kNodes]
   double* mars = vals + (size_t)Q * kNodes;
-  for (int pq = tid; pq < Q * kNodes; pq += kThreads) {
-    const int q = pq / kNodes, jj = pq - q * kNodes, j = j0 + jj;
-    if (j >= n_s) continue;
+  // A work unit is kRun consecutive savings nodes of one child row: the
+  // queries rise with the node, so each bracket search gallops from the
+  // previous node's answer instead of starting over.
+  constexpr int kRun = 4;
+  constexpr int kUnitsPerRow = kNodes / kRun;
+  for (int u = tid; u < Q * kUnitsPerRow; u += kThreads) {
+    const int q = u / kUnitsPerRow, jj0 = (u - q * kUnitsPerRow) * kRun;
+    if (j0 + jj0 >= n_s) continue;
     int t = 0, base = 0;
     for (; t < C.n_targets; ++t) {
       if (C.tkind[t] == 0) continue;
@@ -404,12 +409,16 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
     const LcmEgmRow& rw = C.tRows[t][q - base];
     const int valid = *rw.n_valid;
     int hint = valid >> 1;
-    double val, mar;
-    read_pair(C.tR[t][j], rw, valid, hint, val, mar);
-    mar = mar * C.tG[t][j];
-    if (isneginf_(val)) mar = 0.0;
-    vals[pq] = val;
-    mars[pq] = mar;
+    for (int jj = jj0; jj < jj0 + kRun; ++jj) {
+      const int j = j0 + jj;
+      if (j >= n_s) break;
+      double val, mar;
+      read_pair(C.tR[t][j], rw, valid, hint, val, mar);
+      mar = mar * C.tG[t][j];
+      if (isneginf_(val)) mar = 0.0;
+      vals[q * kNodes + jj] = val;
+      mars[q * kNodes + jj] = mar;
+    }
   }
   __syncthreads();
   const int j = j0 + tid;
@@ -894,6 +903,7 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
                               int max_s) {
   if (!ctx || (!d_cols && n_cols)) return fail(LCM_E_ARG, "lcm_egm_launch: null argument");
   if (n_cols <= 0) return LCM_OK;
+  if (n_cols > 65535) return fail(LCM_E_ARG, "lcm_egm_launch: more than 65535 columns in one launch");
   if (lds_bytes > lcm_egm_lds_bytes(LCM_EGM_MAX_CAND))
     return fail(LCM_E_ARG, "lcm_egm_launch: LDS request beyond the candidate limit");
   int dev = 0, max_lds = 0;
