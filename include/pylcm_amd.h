@@ -335,7 +335,8 @@ typedef struct LcmEgmRow {
  * Outputs: the refined carry rows (n_pad each), counts[0] = kept envelope
  * points (> n_pad: overflow, rows poisoned), counts[1] = valid grid entries,
  * and the published V on the state grid W. cand is scratch of
- * 4 * (n_con + n_s) + 1 doubles.
+ * 4 * (n_con + n_s) + 2 doubles (candidate rows, EV at the first node, and
+ * an all-dead flag handed from the envelope pass to the publishing pass).
  */
 typedef struct LcmEgmCol {
   const double* A;
@@ -386,8 +387,9 @@ typedef struct LcmEgmAgg {
 } LcmEgmAgg;
 
 /* One call solves n_cols columns: a continuation/Euler kernel over blocks of
- * 64 savings nodes, then one workgroup per column for the envelope and the
- * publish. lds_bytes from lcm_egm_lds_bytes(max candidates of any column);
+ * 64 savings nodes, one workgroup per column for the envelope, then wide
+ * kernels (many workgroups per column) for the marginal row and the V row on
+ * W. lds_bytes from lcm_egm_lds_bytes(max candidates of any column);
  * max_rows = the most child rows any column reads (sum over its targets of
  * nodes x rows, <= LCM_EGM_MAX_ROWS_PER_COL); max_s = the most savings nodes.
  * Columns of one call must not read each other's outputs. */

@@ -321,7 +321,8 @@ class EgmProgram:
             if info["kind"] == "rows":
                 self._rowtab[key] = ar.reserve(len(info["rows"]) * ROW.itemsize)
         self._period = {}
-        self._cand_per_col = ((4 * self.max_cand + 1) * 8 + 255) // 256 * 256
+        # candidate rows g, p, v, s of every column, then EV and the all-dead flag
+        self._cand_per_col = ((4 * self.max_cand + 2) * 8 + 255) // 256 * 256
         max_cols = max((sum(s["n_cols"] for s in specs) for specs in self._col_specs.values()), default=0)
         self.cand_off = ar.reserve(max(max_cols, 1) * max(self._cand_per_col, 8))
         for t, specs in self._col_specs.items():

@@ -370,6 +370,8 @@ __device__ __forceinline__ void aggregate_rows(const double* vals, const double*
 }
 
 constexpr int kNodes = 64;  // savings nodes per block of the Euler kernel
+constexpr int kMarginalBlocks = 16;  // workgroups per column for the marginal row (4096 slots)
+constexpr int kPublishBlocks = 8;    // workgroups per column for the V row (2048 wealth nodes)
 
 // Phase 1: continuation and Euler inversion. Block (bx, col) takes savings
 // nodes [64 bx, 64 bx + 64) of column col; its 256 threads read every (child
@@ -806,13 +808,26 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
   }
   __syncthreads();
 
-  // ---- 6. marginal row and publish --------------------------------------------
-  const int n_kept = sh_nkept;
+  // the marginal row and the column's V on the wealth grid are filled by
+  // egm_marginal_kernel and egm_publish_kernel, many workgroups per column
+  if (tid == 0) cg[4 * n + 1] = sh_live == 0 ? 1.0 : 0.0;
+}
+
+// Phase 6a of a column: marginal-utility row of the kept points, NaN past them.
+__global__ void __launch_bounds__(kThreads) egm_marginal_kernel(const LcmEgmCol* __restrict__ cols) {
+  const LcmEgmCol& C = cols[blockIdx.y];
+  const int n = C.n_s + C.n_con;
+  const double* cg = C.cand;
+  const int n_pad = C.n_pad;
+  const int n_kept = C.out_counts[0];
   const int n_valid = min(n_kept, n_pad);
   const bool over = n_kept > n_pad;
-  const bool all_dead = sh_live == 0;
+  const bool all_dead = cg[4 * n + 1] != 0.0;
+  double* og = C.out_grid;
+  double* op = C.out_policy;
+  double* ov = C.out_value;
   double* om = C.out_marg;
-  for (int i = tid; i < n_pad; i += kThreads) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n_pad; i += gridDim.x * kThreads) {
     if (i >= n_valid) {
       og[i] = qnan();
       op[i] = qnan();
@@ -832,12 +847,27 @@ __global__ void __launch_bounds__(kThreads) egm_column_kernel(const LcmEgmCol* _
     }
     om[i] = mu;
   }
-  __syncthreads();
+}
+
+// Phase 6b of a column: its value on the wealth grid (interpolated envelope,
+// or the constrained branch below the first kept point).
+__global__ void __launch_bounds__(kThreads) egm_publish_kernel(const LcmEgmCol* __restrict__ cols) {
+  const LcmEgmCol& C = cols[blockIdx.y];
+  const int n = C.n_s + C.n_con;
+  const double* cg = C.cand;
+  const int n_pad = C.n_pad;
+  const int n_kept = C.out_counts[0];
+  const int n_valid = min(n_kept, n_pad);
+  const bool over = n_kept > n_pad;
+  const bool all_dead = cg[4 * n + 1] != 0.0;
+  const double limit = C.A[0];
+  const double dev0 = C.beta * cg[4 * n];
+  const double* og = C.out_grid;
+  const double* ov = C.out_value;
+  const double* om = C.out_marg;
   const double g0 = og[0];
-  const int w_per = (C.n_w + kThreads - 1) / kThreads;
-  const int w_end = min(C.n_w, (tid + 1) * w_per);
   int w_hint = 0;
-  for (int w = tid * w_per; w < w_end; ++w) {
+  for (int w = blockIdx.x * kThreads + threadIdx.x; w < C.n_w; w += gridDim.x * kThreads) {
     const double x = C.W[w];
     double vint;
     if (isnan_(x) || n_valid == 0) {
@@ -918,6 +948,12 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
                      dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
   hipLaunchKernelGGL(egm::egm_column_kernel, dim3(n_cols), dim3(egm::kThreads), lds_bytes, ctx->stream, d_cols);
+  LCM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(egm::egm_marginal_kernel, dim3(egm::kMarginalBlocks, n_cols), dim3(egm::kThreads), 0,
+                     ctx->stream, d_cols);
+  LCM_HIP(hipGetLastError());
+  hipLaunchKernelGGL(egm::egm_publish_kernel, dim3(egm::kPublishBlocks, n_cols), dim3(egm::kThreads), 0,
+                     ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
   return LCM_OK;
 }
