@@ -167,6 +167,11 @@ typedef struct LcmSimDesc {
   int32_t g_kind, g_n, n_regimes, regime_code;
   int32_t proc_kind, n_rows; /* n_rows = rows of EV (flat discrete states incl. process) */
   double beta, g_start, g_stop, pa, pb, pc;
+  /* > 0: EV1 taste shocks; the discrete action is the first argmax of
+   * Qc_d + taste_scale * (Gumbel - Euler's gamma) over the discrete cells d,
+   * Qc_d the masked max over the continuous actions
+   * (src/_lcm/regime_building/max_Q_over_a.py:255-346) */
+  double taste_scale;
 } LcmSimDesc;
 
 /* Agent panel state (device arrays of n agents). */

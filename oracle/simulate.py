@@ -13,8 +13,9 @@ streams cannot be reproduced): Philox4x32-10 (Salmon et al., SC'11), restated
 here independently of the device code and pinned by the Random123 known-answer
 vectors, with counter (agent, period, stream, 0) and key (seed low, seed high);
 stream 0 draws the regime, 1 the process innovation (Box-Muller), 2 the
-discrete states. Parity of a whole path is therefore exact in the integer
-draws and within float tolerance in the continuous states.
+discrete states, 3 the taste shocks (counter word 3 = the discrete cell).
+Parity of a whole path is therefore exact in the integer draws and within
+float tolerance in the continuous states.
 """
 
 from __future__ import annotations
@@ -50,10 +51,22 @@ def uniform53(a, b):
     return ((a >> np.uint64(5)).astype(np.float64) * 67108864.0 + (b >> np.uint64(6)).astype(np.float64)) / 9007199254740992.0
 
 
-def draws(agents, period, stream, seed):
+def draws(agents, period, stream, seed, word3=0):
     ctr = np.stack([np.asarray(agents, dtype=np.uint64), np.full(len(agents), period, np.uint64),
-                    np.full(len(agents), stream, np.uint64), np.zeros(len(agents), np.uint64)], axis=1)
+                    np.full(len(agents), stream, np.uint64), np.full(len(agents), word3, np.uint64)], axis=1)
     return philox4x32_10(ctr, (seed & MASK, (seed >> 32) & MASK))
+
+
+EULER_GAMMA = 0.5772156649015329
+
+
+def taste_noise(agents, period, d, seed, scale):
+    """Mean-zero EV1 noise of discrete cell d: scale * (Gumbel - Euler's gamma)
+    (src/_lcm/regime_building/max_Q_over_a.py:326-346), the Gumbel drawn as
+    -log(-log(u)) from stream 3 with counter word 3 = d, u in (0, 1)."""
+    blk = draws(agents, period, 3, seed, word3=d)
+    u = uniform53(blk[:, 0], blk[:, 1]) + 2.0 ** -54
+    return scale * (-np.log(-np.log(u)) - EULER_GAMMA)
 
 
 def normal_bm(block):
@@ -119,7 +132,8 @@ def simulate(model, params, V, initial, seed, *, grid_dtype=np.float64):
             idx = np.nonzero((regime == names.index(r)) & (start <= t))[0]
             if idx.size == 0:
                 continue
-            res = _regime_period(model, R, reg, t, params, fixed, gdt, V, w[idx], z[idx], sdd[idx])
+            res = _regime_period(model, R, reg, t, params, fixed, gdt, V, w[idx], z[idx], sdd[idx],
+                                 agents=idx, seed=seed)
             a, val, nxt = res
             out["regime"][t, idx] = names.index(r)
             out["action"][t, idx] = a
@@ -163,8 +177,9 @@ def simulate(model, params, V, initial, seed, *, grid_dtype=np.float64):
     return out
 
 
-def _regime_period(model, R, reg, t, params, fixed, gdt, V, w, z, sdd):
-    """Argmax over actions for agents (arrays w, z, sdd) of one regime."""
+def _regime_period(model, R, reg, t, params, fixed, gdt, V, w, z, sdd, agents=None, seed=0, return_q=False):
+    """Argmax over actions for agents (arrays w, z, sdd) of one regime
+    (with `return_q`: Q and F over the agents' action grids instead)."""
     names = list(model.regime_id_class.__categories__)
     n = len(w)
     s_d, proc = _axes(reg)
@@ -300,12 +315,27 @@ def _regime_period(model, R, reg, t, params, fixed, gdt, V, w, z, sdd):
         CE = CE / np.where(ok, mass, np.nan)
         Q = U + beta * CE
         nxt = [preg, wd, wn_all]
+    if return_q:
+        return Q, F
     Qa = np.where(F, Q, -np.inf).reshape(n, -1)
     Fa = F.reshape(n, -1)
     mx = Qa.max(axis=1)
     nanm = np.any(Fa & np.isnan(Q.reshape(n, -1)), axis=1)
     a = np.argmax((Q.reshape(n, -1) == mx[:, None]) & Fa, axis=1)
     val = np.where(nanm, np.nan, mx)
+    if reg.regime.taste_shocks is not None and not reg.terminal:
+        # Gumbel-max over the discrete cells of the continuous maxima
+        # (max_Q_over_a.py:282-304): Qc per cell, its first maximiser (the
+        # first NaN if any: numpy's argmax), noise per cell, first argmax
+        scale = float(params[reg.name]["taste_shocks"]["scale"])
+        n_dd = int(np.prod(da_sizes)) if da_sizes else 1
+        Qm = Qa.reshape(n, n_dd, -1)
+        Qc = Qm.max(axis=2)
+        ci = np.argmax(Qm, axis=2)
+        noisy = Qc + np.stack([taste_noise(agents, t, dd, seed, scale) for dd in range(n_dd)], axis=1)
+        dsel = np.argmax(noisy, axis=1)
+        a = dsel * Qm.shape[2] + ci[np.arange(n), dsel]
+        val = Qc[np.arange(n), dsel]
     if reg.terminal:
         return a, val, None
     preg, wd, wn_all = nxt

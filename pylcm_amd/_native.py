@@ -95,6 +95,7 @@ class LcmSimDesc(C.Structure):
         ("proc_kind", C.c_int32), ("n_rows", C.c_int32),
         ("beta", C.c_double), ("g_start", C.c_double), ("g_stop", C.c_double),
         ("pa", C.c_double), ("pb", C.c_double), ("pc", C.c_double),
+        ("taste_scale", C.c_double),
     ]
 
 

@@ -385,9 +385,6 @@ class SimProgram:
         self.sl = {r: SimLayout(model.layouts[r]) for r in model.regime_names}
         if len(model.regime_names) > nat.SIM_MAX_REGIMES:
             raise UnsupportedModelError(f"simulation supports at most {nat.SIM_MAX_REGIMES} regimes")
-        for r, sl in self.sl.items():
-            if model.layouts[r].regime.taste_shocks is not None:
-                raise UnsupportedModelError("simulation of taste-shock regimes is not implemented")
 
     def run(self, ic, seed):
         self.prepare(ic, seed)
@@ -643,9 +640,11 @@ class SimProgram:
             kind, a, b, c = draw_params(proc, process_runtime_params(self.params, model.fixed_params, r,
                                                                      sl.proc_name))
             d["proc_kind"], d["pa"], d["pb"], d["pc"] = kind, a, b, c
+        d["taste_scale"] = 0.0
         if not st.terminal:
             pt = self.prog.problems[0][(t, r)]
             d["beta"] = pt.beta
+            d["taste_scale"] = pt.taste_scale or 0.0
             d["preg"] = ar.add(st.preg, dtype=dt)
             d["Wd"] = [None] * nat.SIM_MAX_REGIMES
             d["n_sdd_next"] = [0] * nat.SIM_MAX_REGIMES

@@ -2,6 +2,7 @@
 // EV contraction, the agent partition by regime and the per-agent argmax and
 // state transition. Declarations and reference citations: include/pylcm_amd.h.
 
+#include <climits>
 #include <type_traits>
 
 #include "common.h"
@@ -55,6 +56,8 @@ __device__ __forceinline__ double normal_bm(const U4& r) {
 constexpr uint32_t kStreamRegime = 0;
 constexpr uint32_t kStreamProcess = 1;
 constexpr uint32_t kStreamDiscrete = 2;
+constexpr uint32_t kStreamTaste = 3;  // counter word 3 = the discrete cell
+constexpr double kEulerGamma = 0.5772156649015329;
 
 __global__ void philox_kernel(uint32_t* out, int n, uint32_t c1, uint32_t c2, uint32_t c3,
                               uint32_t k0, uint32_t k1) {
@@ -202,6 +205,26 @@ __device__ __forceinline__ SimBest<T> sim_merge(SimBest<T> a, SimBest<T> b) {
   return a;
 }
 
+// Merge of the kSimLanes lanes of one agent (butterfly within the aligned group).
+template <typename T>
+__device__ __forceinline__ SimBest<T> sim_merge_lanes(SimBest<T> mb) {
+#pragma unroll
+  for (int off = kSimLanes / 2; off >= 1; off >>= 1) {
+    SimBest<T> o;
+    o.best = __shfl_xor(mb.best, off);
+    o.idx = __shfl_xor(mb.idx, off);
+    o.nan = __shfl_xor(int(mb.nan), off) != 0;
+    mb = sim_merge(mb, o);
+  }
+  return mb;
+}
+
+__device__ __forceinline__ int min_lanes(int v) {
+#pragma unroll
+  for (int off = kSimLanes / 2; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T apply_op(int op, T x) {
   switch (op) {
@@ -305,6 +328,14 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   T best = -INFINITY;
   int bidx = -1;
   bool nanf = false;
+  // taste shocks: per discrete cell the masked max Qc and its first maximiser
+  // (the first feasible NaN when Qc is NaN, index 0 of the cell when nothing
+  // is feasible); the cell is the first argmax of Qc + noise
+  const double taste = D->taste_scale;
+  int nan_idx = INT_MAX;  // first feasible NaN of this lane (taste shocks)
+  double t_noisy = 0.0;
+  T t_qc = T(0);
+  int t_idx = -1;
   for (int d = 0; d < n_d; ++d) {
     __syncthreads();
     int nonfinite = 0;
@@ -392,7 +423,9 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
           q = u + beta * cs;
         }
         if (feas) {
-          nanf |= q != q;
+          const bool isn = q != q;
+          nanf |= isn;
+          if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
           if (q == q && (bidx < 0 || q > best)) {
             best = q;
             bidx = d * n_c + c0 + i;
@@ -400,27 +433,59 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
         }
       }
     }
+    if (taste > 0.0) {
+      const SimBest<T> mb = sim_merge_lanes(SimBest<T>{best, bidx, nanf});
+      const int first_nan = min_lanes(nan_idx);
+      T qc;
+      int ci;
+      if (mb.nan) {
+        qc = T(NAN);
+        ci = first_nan;
+      } else if (mb.idx >= 0) {
+        qc = mb.best;
+        ci = mb.idx;
+      } else {
+        qc = T(-INFINITY);
+        ci = d * n_c;
+      }
+      const U4 rt = philox4x32_10(U4{uint32_t(agent), uint32_t(period), kStreamTaste, uint32_t(d)}, k0, k1);
+      const double u = uniform53(rt.x, rt.y) + 0x1p-54;  // in (0, 1)
+      const double noisy = double(qc) + taste * (-log(-log(u)) - kEulerGamma);
+      // first argmax; a NaN counts as the maximum (jnp.argmax)
+      if (t_idx < 0 || (t_noisy == t_noisy && (noisy != noisy || noisy > t_noisy))) {
+        t_noisy = noisy;
+        t_qc = qc;
+        t_idx = ci;
+      }
+      best = T(-INFINITY);
+      bidx = -1;
+      nanf = false;
+      nan_idx = INT_MAX;
+    }
   }
-  // merge the agent's lanes (butterfly within the aligned group of kSimLanes)
-  SimBest<T> mb{best, bidx, nanf};
-#pragma unroll
-  for (int off = kSimLanes / 2; off >= 1; off >>= 1) {
-    SimBest<T> o;
-    o.best = __shfl_xor(mb.best, off);
-    o.idx = __shfl_xor(mb.idx, off);
-    o.nan = __shfl_xor(int(mb.nan), off) != 0;
-    mb = sim_merge(mb, o);
+  if (taste > 0.0) {
+    if (!valid || lane != 0) return;
+  } else {
+    // merge the agent's lanes (butterfly within the aligned group of kSimLanes)
+    const SimBest<T> mb = sim_merge_lanes(SimBest<T>{best, bidx, nanf});
+    best = mb.best;
+    bidx = mb.idx;
+    nanf = mb.nan;
+    if (!valid || lane != 0) return;
   }
-  best = mb.best;
-  bidx = mb.idx;
-  nanf = mb.nan;
-  if (!valid || lane != 0) return;
 
   // record this period
   const int64_t o = int64_t(period) * n_agents + agent;
   out.regime[o] = r;
-  out.action[o] = (bidx >= 0 && !nanf) ? bidx : 0;
-  static_cast<T*>(out.value)[o] = nanf ? T(NAN) : (bidx >= 0 ? best : T(-INFINITY));
+  if (taste > 0.0) {
+    // the drawn cell's maximiser and its value without the noise
+    bidx = t_idx;
+    out.action[o] = t_idx;
+    static_cast<T*>(out.value)[o] = t_qc;
+  } else {
+    out.action[o] = (bidx >= 0 && !nanf) ? bidx : 0;
+    static_cast<T*>(out.value)[o] = nanf ? T(NAN) : (bidx >= 0 ? best : T(-INFINITY));
+  }
   static_cast<T*>(out.w)[o] = w;
   static_cast<T*>(out.z)[o] = z;
   out.sdd[o] = sdd;

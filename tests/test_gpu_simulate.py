@@ -118,3 +118,53 @@ def test_simulate_large_panel_f32(native):
     assert np.allclose(raw["value"][0], V[0]["working_life"][zi, wi], rtol=1e-6)
     z1 = raw["z"][1].astype(np.float64)
     assert abs(z1.mean() - 0.95 * z0.mean()) < 0.005
+
+
+def test_simulate_taste_shocks_f64(native):
+    """EV1 taste shocks on the work/retire choice: the device's Gumbel-max
+    draws equal the oracle's (same Philox stream), so paths are identical."""
+    T = 6
+    m = tm.taste_shock_model(T)
+    p = ex.retirement_params(T, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0})
+    p["working_life"]["taste_shocks"] = {"scale": 0.3}
+    rng = np.random.default_rng(4)
+    n = 300
+    ic = {"wealth": rng.uniform(1, 400, n), "income": rng.normal(0, 0.2, n),
+          "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
+    res = _compare(m, p, ic, np.float64, seed=21, rtol=1e-9)
+    # both discrete choices occur (the draw is not the plain argmax)
+    d = res.raw["action"][0] // 40
+    assert 0 < d.mean() < 1
+
+
+def test_simulate_taste_shock_choice_frequencies_f32(native):
+    """Many agents at one state: the share choosing each discrete action is
+    the logit of the cells' continuous maxima (McFadden), computed from the
+    oracle's Q at that state."""
+    T = 4
+    m = tm.taste_shock_model(T)
+    p = ex.retirement_params(T, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0})
+    scale = 0.2
+    p["working_life"]["taste_shocks"] = {"scale": scale}
+    V, _ = og.solve(m, p, grid_dtype=np.float32)
+    n = 100_000
+    w0 = np.full(n, 37.0)
+    z0 = np.zeros(n)
+    ic = {"wealth": w0, "income": z0, "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=3, dtype=np.float32)
+    d = res.raw["action"][0] // 40
+    freq = np.array([np.mean(d == 0), np.mean(d == 1)])
+    one = {k: v[:1] for k, v in ic.items()}
+    R = {nm: og._R(nm, m.regimes[nm], np.float32, p, m.fixed_params) for nm in m.regime_id_class.__categories__}
+    Q, F = _period0_q(m, R, p, V, one)
+    Qc = np.where(F, Q, -np.inf).reshape(2, -1).max(axis=1)
+    logit = np.exp((Qc - Qc.max()) / scale) / np.exp((Qc - Qc.max()) / scale).sum()
+    print(f"frequencies {freq}, logit {logit}")
+    assert np.max(np.abs(freq - logit)) < 0.01
+
+
+def _period0_q(m, R, p, V, one):
+    """Q and feasibility over (d, c) for one agent in period 0 (oracle)."""
+    return osim._regime_period(m, R, R["working_life"], 0, p, m.fixed_params, np.float32, V,
+                               np.asarray(one["wealth"], float), np.asarray(one["income"], float),
+                               np.zeros(1, dtype=np.int64), return_q=True)

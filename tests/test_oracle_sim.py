@@ -68,3 +68,23 @@ def test_stochastic_process_path_moments():
     out = osim.simulate(m, p, V, ic, seed=3)
     z1 = out["z"][1]
     assert abs(z1.mean()) < 0.01 and abs(z1.std() - 0.1) < 0.005
+
+
+def test_taste_noise_is_mean_zero_gumbel():
+    """scale * (Gumbel - Euler's gamma): mean 0, variance scale^2 pi^2 / 6."""
+    n = 200_000
+    noise = osim.taste_noise(np.arange(n), 3, 0, 12345, 0.7)
+    assert abs(noise.mean()) < 0.01
+    assert abs(noise.var() / (0.7 ** 2 * np.pi ** 2 / 6) - 1) < 0.02
+
+
+def test_gumbel_max_choice_frequencies_are_logit():
+    """The first argmax of Qc + noise picks cell d with probability
+    softmax(Qc / scale)_d (McFadden), the closed form the smoothed solve uses."""
+    n = 200_000
+    Qc = np.array([0.3, 0.1, -0.2])
+    agents = np.arange(n)
+    noisy = Qc[None, :] + np.stack([osim.taste_noise(agents, 5, d, 99, 0.5) for d in range(3)], axis=1)
+    freq = np.bincount(np.argmax(noisy, axis=1), minlength=3) / n
+    logit = np.exp(Qc / 0.5) / np.exp(Qc / 0.5).sum()
+    assert np.max(np.abs(freq - logit)) < 0.005
