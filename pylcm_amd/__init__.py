@@ -11,6 +11,7 @@ from pylcm_amd import config
 from pylcm_amd.ages import AgeGrid
 from pylcm_amd.exceptions import (
     GridInitializationError,
+    InvalidAdditionalTargetsError,
     InvalidInitialConditionsError,
     InvalidParamsError,
     InvalidValueFunctionError,
@@ -58,7 +59,8 @@ class LinearExpectation:
 
 __all__ = [
     "AgeGrid", "ConsumptionSavingsRegime", "DCEGM", "DiscreteGrid", "ExtremeValueTasteShocks",
-    "FUESEnvelope", "GridInitializationError", "GridSearch", "InvalidInitialConditionsError",
+    "FUESEnvelope", "GridInitializationError", "GridSearch", "InvalidAdditionalTargetsError",
+    "InvalidInitialConditionsError",
     "InvalidParamsError", "InvalidValueFunctionError", "IrregSpacedGrid", "LinSpacedGrid",
     "LinearAggregator", "LinearExpectation", "LiquidMargin", "LogNormalIIDProcess",
     "LogSpacedGrid", "MarkovTransition", "Model", "ModelInitializationError",

@@ -285,16 +285,77 @@ class SimLowering(PeriodLowering):
 class SimulationResult:
     """Simulated panel (`src/lcm/result.py:125-185`)."""
 
-    def __init__(self, frame: pd.DataFrame, model, terminal_names):
+    def __init__(self, frame: pd.DataFrame, model, terminal_names, params=None, dtype=None):
         self._frame = frame
         self._model = model
         self._terminal = terminal_names
+        self._params = params
+        self._dtype = dtype
+
+    # -- additional targets (src/_lcm/simulation/additional_targets.py:54-265) --
+    def _targets_of(self, r):
+        """User functions and constraints of regime r that can be evaluated
+        from a row of the panel (the inverse marginal utility of a DC-EGM
+        regime reads the continuation, so it is not one)."""
+        lay = self._model.layouts[r]
+        reg = lay.regime
+        names = {k for k, f in reg.functions.items() if f is not None} | set(reg.constraints)
+        if lay.is_dcegm:
+            names.discard("inverse_marginal_utility")
+        return names
+
+    @property
+    def available_targets(self) -> list[str]:
+        """Names accepted by `to_dataframe(additional_targets=...)`."""
+        out = set()
+        for r in self._model.regime_names:
+            out |= self._targets_of(r)
+        return sorted(out)
+
+    def _compute_targets(self, df, targets):
+        from pylcm_amd._dag import Evaluator
+        from pylcm_amd.exceptions import InvalidAdditionalTargetsError
+
+        if targets == "all":
+            targets = self.available_targets
+        bad = set(targets) - set(self.available_targets)
+        if bad:
+            raise InvalidAdditionalTargetsError(
+                f"Targets {bad} not found in any regime. Available targets: {self.available_targets}")
+        cols = {t: np.full(len(df), np.nan, dtype=object) for t in targets}
+        model = self._model
+        for r in model.regime_names:
+            rows = np.nonzero((df["regime_name"] == r).to_numpy())[0]
+            mine = [t for t in targets if t in self._targets_of(r)]
+            if not rows.size or not mine:
+                continue
+            lay = model.layouts[r]
+            reg = lay.regime
+            variables = {"age": df["age"].to_numpy()[rows], "period": df["period"].to_numpy()[rows]}
+            for name in list(lay.state_names) + list(lay.action_names):
+                v = df[name].to_numpy()[rows]
+                discrete = name in lay.sd_names and isinstance(lay.sd_grids[lay.sd_names.index(name)], DiscreteGrid)
+                discrete |= name in lay.da_names
+                variables[name] = v.astype(np.int64) if discrete else v.astype(np.float64)
+            funcs = {k: f for k, f in reg.functions.items() if f is not None}
+            ev = Evaluator(regime_name=r, functions=funcs, variables=variables, params=self._params,
+                           fixed_params=model.fixed_params, param_dtype=self._dtype)
+            for t in mine:
+                val = ev.call(t, reg.constraints[t]) if t in reg.constraints else ev.call(t)
+                cols[t][rows] = np.broadcast_to(np.asarray(val), rows.shape)
+        out = df.copy()
+        for t, v in cols.items():
+            try:
+                out[t] = v.astype(np.float64)
+            except (TypeError, ValueError):
+                out[t] = v
+        return out
 
     def to_dataframe(self, additional_targets=None, *, use_labels: bool = True,
                      terminal_rows: str = "first") -> pd.DataFrame:
-        if additional_targets:
-            raise NotImplementedError("additional_targets are not implemented")
         df = self._frame
+        if additional_targets:
+            df = self._compute_targets(df, additional_targets)
         if terminal_rows == "first" and self._terminal:
             is_term = df["regime_name"].isin(list(self._terminal))
             first = df[is_term].groupby("subject_id")["period"].transform("min")
@@ -583,7 +644,8 @@ class SimProgram:
         out.update({k: get(k, dt) for k in ("value", "w", "z")})
         self.raw = out
         res = SimulationResult(self._frame(out, ic), model,
-                               frozenset(r for r in names if model.layouts[r].terminal))
+                               frozenset(r for r in names if model.layouts[r].terminal),
+                               params=self.params, dtype=self.dt)
         res.raw = out  # [period, agent] arrays as the device wrote them
         return res
 

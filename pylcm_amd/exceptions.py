@@ -54,3 +54,8 @@ class NativeKernelError(PyLCMError):
     def __init__(self, msg, status=None):
         super().__init__(msg)
         self.status = status
+
+
+class InvalidAdditionalTargetsError(PyLCMError):
+    """`to_dataframe(additional_targets=...)` names a target no regime defines
+    (`src/lcm/exceptions.py` InvalidAdditionalTargetsError)."""

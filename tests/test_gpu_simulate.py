@@ -69,6 +69,14 @@ def test_simulate_deterministic_f64(native):
     df = res.to_dataframe()
     assert list(df.columns[:4]) == ["subject_id", "period", "regime_name", "value"]
     assert set(df["regime_name"].unique()) <= {"working_life", "retirement", "dead"}
+    # every row of an alive regime with a feasible action (finite value)
+    # satisfies the borrowing constraint and has a finite utility (additional
+    # targets evaluated at the panel); rows with no feasible action hold -inf
+    tg = res.to_dataframe(additional_targets=["borrowing_constraint", "utility"], use_labels=False)
+    ok = (tg["regime_name"] != list(m.regime_names).index("dead")) & np.isfinite(tg["value"])
+    assert ok.sum() > 0
+    assert (tg.loc[ok, "borrowing_constraint"] == 1).all()
+    assert np.isfinite(tg.loc[ok, "utility"].astype(float)).all()
 
 
 def test_simulate_stochastic_f64(native):
