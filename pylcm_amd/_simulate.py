@@ -392,11 +392,66 @@ def _discrete_labels(model):
 # ---------------------------------------------------------------------------
 
 
+DCEGM_BUDGET_CONSTRAINT_NAME = "dcegm_budget_constraint"
+
+
+def _budget_constraint(action: str, resources: str, limit: float):
+    """`action <= resources - limit` as a constraint reading those two names
+    (src/_lcm/egm/budget.py:36-95)."""
+    import inspect
+
+    def dcegm_budget_constraint(**kw):
+        return kw[action] <= kw[resources] - limit
+
+    P = inspect.Parameter
+    dcegm_budget_constraint.__signature__ = inspect.Signature(
+        [P(action, P.POSITIONAL_OR_KEYWORD), P(resources, P.POSITIONAL_OR_KEYWORD)])
+    return dcegm_budget_constraint
+
+
+def grid_argmax_view(model):
+    """The model with every DC-EGM regime simulated by the argmax over its own
+    action grids, as the reference's simulation does for every shipped
+    envelope (src/_lcm/regime_building/processing.py:1662-1700): the regime
+    keeps its functions (minus the inverse marginal utility, which only the
+    Euler inversion reads) and gains the budget mask the EGM solve enforces
+    intrinsically, `action <= resources - savings_grid[0]`
+    (src/_lcm/egm/budget.py:36-95)."""
+    from pylcm_amd.model import Model
+    from pylcm_amd.regime import Regime
+    from pylcm_amd.solvers import DCEGM, GridSearch
+
+    regimes = {}
+    for name, reg in model.user_regimes.items():
+        if not isinstance(reg.solver, DCEGM) or reg.terminal:
+            regimes[name] = reg
+            continue
+        mg = reg.liquid
+        limit = float(reg.solver.savings_grid.to_numpy(np.float64)[0])
+        regimes[name] = Regime(
+            transition=reg.transition, active=reg.active, states=reg.states,
+            state_transitions=reg.state_transitions, actions=reg.actions,
+            functions={k: f for k, f in reg.functions.items() if k != "inverse_marginal_utility"},
+            constraints={**reg.constraints,
+                         DCEGM_BUDGET_CONSTRAINT_NAME: _budget_constraint(mg.action, mg.resources, limit)},
+            derived_categoricals=reg.derived_categoricals, solver=GridSearch(),
+            taste_shocks=reg.taste_shocks, koopmans_aggregator=reg.koopmans_aggregator,
+            certainty_equivalent=reg.certainty_equivalent)
+    return Model(regimes=regimes, ages=model.ages, regime_id_class=model.regime_id_class,
+                 fixed_params=model.fixed_params)
+
+
 def simulate_model(model, *, params, initial_conditions, V, seed, dtype, subject_batch_size=0,
                    log_level="off", prog: GsProgram | None = None):
     dt = np.dtype(dtype)
     if any(lay.is_dcegm for lay in model.layouts.values()):
-        raise UnsupportedModelError("simulation of DC-EGM regimes is not implemented")
+        # DC-EGM regimes: value functions from the DC-EGM solve, choices by
+        # the grid argmax of the view
+        if V is None:
+            from pylcm_amd._egm_program import solve_dcegm_model
+
+            V, _ = solve_dcegm_model(model, params, dt)
+        model = grid_argmax_view(model)
     ic = _initial_conditions(model, initial_conditions)
     if prog is None:
         prog = GsProgram(model, [params], dt)

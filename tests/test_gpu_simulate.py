@@ -34,11 +34,13 @@ def test_device_philox_matches_oracle(native):
     assert [int(x) for x in got] == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
 
 
-def _compare(model, params, ic, dtype, seed, rtol):
-    V, _ = og.solve(model, params, grid_dtype=dtype)
+def _compare(model, params, ic, dtype, seed, rtol, V=None, oracle_model=None):
+    if V is None:
+        V, _ = og.solve(model, params, grid_dtype=dtype)
     res = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V,
                          seed=seed, dtype=dtype)
     got = res.raw
+    model = oracle_model or model
     exp = osim.simulate(model, params, V, ic, seed, grid_dtype=dtype)
     sim = exp["action"] >= 0
     assert np.array_equal(got["action"] >= 0, sim)
@@ -176,3 +178,28 @@ def _period0_q(m, R, p, V, one):
     return osim._regime_period(m, R, R["working_life"], 0, p, m.fixed_params, np.float32, V,
                                np.asarray(one["wealth"], float), np.asarray(one["income"], float),
                                np.zeros(1, dtype=np.int64), return_q=True)
+
+
+def test_simulate_dcegm_model_f64(native):
+    """DC-EGM regimes simulate by the argmax over their own action grids with
+    the intrinsic budget mask, from the DC-EGM value functions (the
+    reference's path for every shipped envelope): paths equal the oracle's
+    simulation of that grid-argmax view on the same V, and every feasible row
+    keeps consumption <= wealth - savings_grid[0]."""
+    from pylcm_amd._simulate import grid_argmax_view
+
+    T = 5
+    m = ex.dcegm_retirement_model(T, n_savings=150, savings_max=100.0, n_wealth=70,
+                                  wealth_range=(0.05, 100.0), n_consumption=90)
+    p = ex.dcegm_params(T)
+    V = m.solve(params=p, log_level="off", dtype=np.float64)
+    rng = np.random.default_rng(6)
+    n = 200
+    ic = {"wealth": rng.uniform(0.5, 95, n), "income": rng.normal(0, 0.1, n),
+          "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    res = _compare(m, p, ic, np.float64, seed=8, rtol=1e-9, V=V, oracle_model=grid_argmax_view(m))
+    df = res.to_dataframe(use_labels=False)
+    alive = df["regime_name"] != list(m.regime_names).index("dead")
+    ok = alive & np.isfinite(df["value"])
+    assert ok.sum() > 0
+    assert (df.loc[ok, "consumption"] <= df.loc[ok, "wealth"] + 1e-12).all()
