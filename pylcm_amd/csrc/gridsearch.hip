@@ -303,6 +303,25 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     best = take ? q : best;
     idx = take ? c : idx;
   };
+  // In the four-action loops the maximiser is kept as (it, slot): the loop
+  // position of the last quad that improved and the position k in that quad,
+  // idx = it + k * SPLIT; the slot select takes an inline constant.
+  int it = idx, slot = 0;
+  auto update4 = [&](T q0, T q1, T q2, T q3, bool f0, bool f1, bool f2, bool f3, int c) {
+    const bool t0 = f0 & !(q0 <= best);
+    best = t0 ? q0 : best;
+    slot = t0 ? 0 : slot;
+    const bool t1 = f1 & !(q1 <= best);
+    best = t1 ? q1 : best;
+    slot = t1 ? 1 : slot;
+    const bool t2 = f2 & !(q2 <= best);
+    best = t2 ? q2 : best;
+    slot = t2 ? 2 : slot;
+    const bool t3 = f3 & !(q3 <= best);
+    best = t3 ? q3 : best;
+    slot = t3 ? 3 : slot;
+    it = (t0 | t1 | t2 | t3) ? c : it;
+  };
   if (__builtin_amdgcn_ballot_w64(L.valid) == 0) return;  // no state of this wave is real
   int i = L.grp;
   if constexpr (LO && !HI) {
@@ -323,10 +342,7 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       const T q1 = q_of(a1);
       const T q2 = q_of(a2);
       const T q3 = q_of(a3);
-      update(q0, L.valid, base + i);
-      update(q1, L.valid, base + i + SPLIT);
-      update(q2, L.valid, base + i + 2 * SPLIT);
-      update(q3, L.valid, base + i + 3 * SPLIT);
+      update4(q0, q1, q2, q3, L.valid, L.valid, L.valid, L.valid, base + i);
     }
   }
   for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
@@ -343,11 +359,9 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     const T q1 = q_of(a1);
     const T q2 = q_of(a2);
     const T q3 = q_of(a3);
-    update(q0, f0, base + i);
-    update(q1, f1, base + i + SPLIT);
-    update(q2, f2, base + i + 2 * SPLIT);
-    update(q3, f3, base + i + 3 * SPLIT);
+    update4(q0, q1, q2, q3, f0, f1, f2, f3, base + i);
   }
+  idx = it + slot * SPLIT;
   for (; i < L.cn; i += SPLIT) {
     const Act<T> a = L.act[i];
     const bool f = feas_of(a, i);
