@@ -29,6 +29,13 @@ from pylcm_amd.grids import (
     categorical,
 )
 from pylcm_amd.model import Model
+from pylcm_amd.persistence import (
+    SimulateSnapshot,
+    SolveSnapshot,
+    load_snapshot,
+    load_solution,
+    save_solution,
+)
 from pylcm_amd.processes import (
     LogNormalIIDProcess,
     NormalIIDProcess,
@@ -66,5 +73,6 @@ __all__ = [
     "LogSpacedGrid", "MarkovTransition", "Model", "ModelInitializationError",
     "NativeKernelError", "NormalIIDProcess", "PyLCMError", "RegimeInitializationError",
     "Regime", "RouwenhorstAR1Process", "TauchenAR1Process", "UniformIIDProcess",
-    "UnsupportedModelError", "categorical", "config",
+    "SimulateSnapshot", "SolveSnapshot", "UnsupportedModelError", "categorical", "config",
+    "load_snapshot", "load_solution", "save_solution",
 ]

@@ -107,8 +107,27 @@ class Model:
 
         dt = np.dtype(dtype) if dtype is not None else config.canonical_float_dtype()
         V, pol = solve_model(self, params, dt)
-        self._check_nan(V, log_level)
+        try:
+            self._check_nan(V, log_level)
+        except InvalidValueFunctionError as exc:
+            # src/lcm/model.py:552-563: snapshot of the partial solution
+            if log_path is not None and exc.partial_solution is not None:
+                from pylcm_amd.persistence import save_solve_snapshot
+
+                snap = save_solve_snapshot(model=self, params=params,
+                                           period_to_regime_to_V_arr=exc.partial_solution,
+                                           log_path=log_path, log_keep_n_latest=log_keep_n_latest)
+                exc.add_note(f"Snapshot saved to {snap}")
+            raise
         Vm = MappingProxyType({t: MappingProxyType(V[t]) for t in sorted(V)})
+        # src/lcm/model.py:565-577: every solve at debug, a NaN solve at
+        # warning / progress
+        if log_path is not None and log_level != "off" and (
+                log_level == "debug" or any(np.isnan(a).any() for t in V for a in V[t].values())):
+            from pylcm_amd.persistence import save_solve_snapshot
+
+            save_solve_snapshot(model=self, params=params, period_to_regime_to_V_arr=Vm,
+                                log_path=log_path, log_keep_n_latest=log_keep_n_latest)
         if return_policy or return_simulation_policy:
             Pm = MappingProxyType({t: MappingProxyType(pol[t]) for t in sorted(pol)})
             return Vm, Pm
@@ -146,9 +165,24 @@ class Model:
         from pylcm_amd._simulate import simulate_model
 
         dt = np.dtype(dtype) if dtype is not None else config.canonical_float_dtype()
-        return simulate_model(self, params=params, initial_conditions=initial_conditions,
-                              V=period_to_regime_to_V_arr, seed=seed, dtype=dt,
-                              subject_batch_size=subject_batch_size, log_level=log_level)
+        snapshot = log_path is not None and log_level == "debug"
+        if snapshot and period_to_regime_to_V_arr is None and not self._any_dcegm():
+            period_to_regime_to_V_arr = self.solve(params=params, dtype=dt)
+        result = simulate_model(self, params=params, initial_conditions=initial_conditions,
+                                V=period_to_regime_to_V_arr, seed=seed, dtype=dt,
+                                subject_batch_size=subject_batch_size, log_level=log_level)
+        if snapshot:
+            # src/lcm/model.py:838-847
+            from pylcm_amd.persistence import save_simulate_snapshot
+
+            save_simulate_snapshot(model=self, params=params, initial_conditions=initial_conditions,
+                                   period_to_regime_to_V_arr=period_to_regime_to_V_arr or {},
+                                   result=result, log_path=log_path,
+                                   log_keep_n_latest=log_keep_n_latest)
+        return result
+
+    def _any_dcegm(self) -> bool:
+        return any(lay.is_dcegm for lay in self.layouts.values())
 
     def get_params_template(self):
         """Parameter names each regime's functions read, by function."""

@@ -1,0 +1,69 @@
+"""Solution files and snapshots around GPU solves and simulations: a saved
+solution warm-starts `simulate` with the same panel as the solution it came
+from, and `log_path` snapshots hold what the run returned."""
+
+import numpy as np
+import pytest
+
+from pylcm_amd import examples as ex
+from pylcm_amd.persistence import SimulateSnapshot, SolveSnapshot, load_snapshot, load_solution, save_solution
+
+pytestmark = pytest.mark.gpu
+
+
+def _model():
+    T = 6
+    m = ex.retirement_model(T, n_wealth=40, n_consumption=50, income_nodes=5)
+    p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.2, "mu": 0.0})
+    rng = np.random.default_rng(8)
+    n = 200
+    ic = {"wealth": rng.uniform(1, 200, n), "income": rng.normal(0, 0.3, n),
+          "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    return m, p, ic
+
+
+def _same_V(a, b):
+    assert set(a) == set(b)
+    for t in a:
+        assert set(a[t]) == set(b[t])
+        for r in a[t]:
+            assert np.array_equal(np.asarray(a[t][r]), np.asarray(b[t][r]), equal_nan=True)
+
+
+def test_saved_solution_warm_starts_simulate(native, tmp_path):
+    m, p, ic = _model()
+    V = m.solve(params=p, dtype=np.float64)
+    path = save_solution(period_to_regime_to_V_arr=V, path=tmp_path / "solution.h5")
+    Vl = load_solution(path=path)
+    _same_V(V, Vl)
+    a = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=4,
+                   dtype=np.float64).raw
+    b = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=Vl, seed=4,
+                   dtype=np.float64).raw
+    for k in ("action", "regime", "value", "w"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+def test_debug_solve_and_simulate_write_snapshots(native, tmp_path):
+    m, p, ic = _model()
+    V = m.solve(params=p, dtype=np.float64, log_level="debug", log_path=tmp_path)
+    snaps = sorted(tmp_path.glob("solve_snapshot_*"))
+    assert [s.name for s in snaps] == ["solve_snapshot_001"]
+    s = load_snapshot(snaps[0])
+    assert isinstance(s, SolveSnapshot)
+    _same_V(V, s.period_to_regime_to_V_arr)
+    # the unpickled model solves to the same V
+    _same_V(V, s.model.solve(params=s.params, dtype=np.float64))
+
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=2,
+                     dtype=np.float64, log_level="debug", log_path=tmp_path)
+    snaps = sorted(tmp_path.glob("simulate_snapshot_*"))
+    assert len(snaps) == 1
+    s = load_snapshot(snaps[0])
+    assert isinstance(s, SimulateSnapshot)
+    _same_V(V, s.period_to_regime_to_V_arr)
+    df0, df1 = res.to_dataframe(), s.result.to_dataframe()
+    assert df0.equals(df1)
+    # off / warning without NaN: no further solve snapshot
+    m.solve(params=p, dtype=np.float64, log_level="warning", log_path=tmp_path)
+    assert len(list(tmp_path.glob("solve_snapshot_*"))) == 1
