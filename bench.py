@@ -307,7 +307,15 @@ def _serve_window(args, rate, seconds, seed):
            "fused_launches": server.launches,
            "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
            "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
-           "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1)}
+           "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
+           "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())}}
+    # where the latency goes: lowering (worker processes), placement (device
+    # memory wait + uploads), and the solve (admission to completion)
+    stages = {"lower": [r.t_lowered - r.t_submit for r in ok],
+              "place": [r.t_admit - r.t_lowered for r in ok],
+              "solve": [r.t_done - r.t_admit for r in ok]}
+    out["stage_ms"] = {k: {"p50": round(1e3 * float(np.percentile(v, 50)), 2),
+                           "p99": round(1e3 * float(np.percentile(v, 99)), 2)} for k, v in stages.items()}
     log(f"serving at {rate:g}/s: {len(ok)}/{len(reqs)} requests in {wall:.1f}s, {out['solves_per_s']:.1f} solves/s, "
         f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {server.launches} fused launches")
     return out
@@ -385,7 +393,7 @@ def main():
     ap.add_argument("--serve-sat-seconds", type=float, default=3.0)
     ap.add_argument("--serve-procs", type=int, default=8)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
-    ap.add_argument("--serve-tile", type=int, default=256)
+    ap.add_argument("--serve-tile", type=int, default=0, help="0: chosen per fused launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

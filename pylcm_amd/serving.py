@@ -120,6 +120,8 @@ class Request:
     slab: tuple | None = None
     desc: tuple | None = None
     tasks: list | None = None
+    desc_base: int = 0
+    n_w: object = None
     done_event: object = None
     future: object = None
     result: dict | None = None
@@ -131,23 +133,33 @@ class Request:
 
 
 class SolveServer:
+    FINE_TILE, COARSE_TILE = 64, 256
+
     """Continuous batching of grid-search solves on one GPU.
 
     `budget_bytes` caps the device memory the server uses (default: the given
-    fraction of the free HBM at start). `tile` is shared by all requests (the
-    fused launch runs one tile shape). `max_inflight` caps admitted requests.
+    fraction of the free HBM at start). `tile` is the state tile of the fused
+    launch (one shape per launch). `tile=0` picks it per launch: a launch with
+    fewer than `fine_below` 256-state tasks splits each into its 64-state
+    tasks (four waves share each task's actions, so a small launch has four
+    times the workgroups); a larger one runs the 256-state tasks (a quarter of
+    the per-task staging). The tile does not change results: the maximum and
+    its first maximiser are merged the same way whatever the split. `max_inflight` caps admitted requests.
     `lower_procs` > 0 lowers requests in that many spawned worker processes
     (models are sent with cloudpickle); as with any spawned pool, the main
     script must guard its entry point with `if __name__ == "__main__":`.
     """
 
-    def __init__(self, *, dtype=np.float32, tile: int = 256, budget_bytes: int | None = None,
+    def __init__(self, *, dtype=np.float32, tile: int = 0, budget_bytes: int | None = None,
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1,
-                 lower_procs: int = 0):
+                 lower_procs: int = 0, fine_below: int = 256):
         self.dt = np.dtype(dtype)
         self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
-        self.tile = int(tile)
+        self.auto_tile = int(tile) == 0
+        self.tile = self.COARSE_TILE if self.auto_tile else int(tile)  # the lowering tile
+        self.fine_below = int(fine_below)
+        self.tiles_used = {}  # tile -> fused launches
         self.fetch_mode = fetch
         self.ctx = nat.Context()        # kernels and task uploads
         self.copy_ctx = nat.Context()   # request images in, results out
@@ -286,6 +298,8 @@ class SolveServer:
                 rec["w0"] = t[:, 2]
             steps.append(rec)
         req.tasks = steps
+        req.desc_base = desc_base
+        req.n_w = np.asarray(D0["n_w"], dtype=np.int64)  # per descriptor of the request
         req.slab, req.desc = sl, ds
         req.t_admit = time.perf_counter()
         req.step = 0
@@ -304,23 +318,31 @@ class SolveServer:
         if self.ring_used[k]:
             self.ring_events[k].synchronize()  # the copy from this host slot has finished
         rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
-        n, lds, variant = 0, 0, 0
+        picks, n_coarse, lds, variant = [], 0, 0, 0
         for req in self.inflight:
             prog = req.prog
             while req.step < prog.n_steps and len(req.tasks[req.step]) == 0:
                 req.step += 1
             if req.step >= prog.n_steps:
                 continue
-            t = req.tasks[req.step]
-            if n + len(t) + 8 > self.max_tasks:
+            tc = req.tasks[req.step]
+            if n_coarse + len(tc) + 8 > self.max_tasks:
                 break  # the rest join the next launch
-            rec[n: n + len(t)] = t
-            n += len(t)
+            picks.append((req, req.step))
+            n_coarse += len(tc)
             lds = max(lds, prog.lds_by_step[req.step])
             variant = max(variant, prog.wdep_by_step[req.step])
             req.step += 1
-        if n == 0:
+        if not picks:
             return False
+        fine = self.auto_tile and n_coarse < self.fine_below and 4 * n_coarse + 8 <= self.max_tasks
+        tile = self.FINE_TILE if fine else self.tile
+        n = 0
+        for req, st in picks:
+            t = self._split(req, req.tasks[st]) if fine else req.tasks[st]
+            rec[n: n + len(t)] = t
+            n += len(t)
+        self.tiles_used[tile] = self.tiles_used.get(tile, 0) + 1
         npad = (n + 7) // 8 * 8
         rec["desc"][n:npad] = -1
         self.ring_i = (k + 1) % self.n_ring
@@ -329,7 +351,7 @@ class SolveServer:
         nat.call("lcm_memcpy_h2d_async", self.ctx.handle, C.c_void_p(self.task_dev[k].ptr),
                  C.c_void_p(self.task_host[k].ptr), C.c_size_t(npad * TASK_DTYPE.itemsize))
         nat.call("lcm_gs_launch_ex", self.ctx.handle, self.code, C.c_void_p(self.desc_buf.ptr),
-                 C.c_void_p(self.task_dev[k].ptr), npad, self.tile, C.c_size_t(lds), int(variant))
+                 C.c_void_p(self.task_dev[k].ptr), npad, tile, C.c_size_t(lds), int(variant))
         self.ring_events[k].record(self.ctx)
         e_end = nat.Event()
         e_end.record(self.ctx)
@@ -348,6 +370,14 @@ class SolveServer:
                 still.append(req)
         self.inflight = still
         return True
+
+    def _split(self, req, t):
+        """The 64-state tasks of 256-state tasks `t`: each tile's four
+        quarters that start below its descriptor's n_w."""
+        k = self.COARSE_TILE // self.FINE_TILE
+        r = np.repeat(t, k)
+        r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * self.FINE_TILE, len(t))
+        return r[r["w0"] < req.n_w[r["desc"] - req.desc_base]]
 
     def _retire(self, block: bool = False):
         still, freed = [], False

@@ -100,3 +100,27 @@ def test_worker_process_lowering_same_results(native):
         for t in ref:
             for r in ref[t]:
                 assert np.array_equal(h.result[t][r], ref[t][r]), (h.rid, t, r)
+
+
+def test_tile_chosen_per_launch_same_results(native):
+    """tile=0: small launches run 64-state tasks, large ones 256-state tasks;
+    every request equals its solve alone with either tile."""
+    from pylcm_amd.serving import SolveServer
+
+    reqs = _requests(20, seed=9, max_grid=600)
+    server = SolveServer(tile=0, fetch="all", budget_bytes=1 << 30, fine_below=24)
+    handles = [server.submit(m, p) for m, p in reqs[:2]]
+    for _ in range(3):
+        server.step()
+    handles += [server.submit(m, p) for m, p in reqs[2:]]
+    server.drain()
+    server.close()
+    assert set(server.tiles_used) == {64, 256}, server.tiles_used
+    for h, (m, p) in zip(handles, reqs):
+        assert h.error is None, h.error
+        ref = _alone(m, p, 256)
+        fine = _alone(m, p, 64)
+        for t in ref:
+            for r in ref[t]:
+                assert np.array_equal(fine[t][r], ref[t][r]), (h.rid, t, r)
+                assert np.array_equal(h.result[t][r], ref[t][r]), (h.rid, t, r)

@@ -102,3 +102,35 @@ def test_worker_process_lowering_matches_in_process():
         assert (got["total_bytes"], got["out_start"], got["n_desc"], got["n_steps"]) == (
             prog.total_bytes, prog.out_start, prog.n_desc, prog.n_steps)
         assert got["outs"][0].keys() == prog.outs[0].keys()
+
+
+def test_split_tasks_are_the_64_state_tiling():
+    """With the tile chosen per launch the server splits a request's
+    256-state tasks into their 64-state quarters for small launches: the same
+    task set as lowering with tile 64."""
+    from types import SimpleNamespace
+
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import TASK_DTYPE, GsProgram
+    from pylcm_amd.serving import SolveServer
+
+    for T, g, nodes in ((5, 700, 3), (4, 257, None), (6, 64, 5), (3, 1000, None)):
+        m = ex.retirement_model(T, n_wealth=g, n_consumption=50, income_nodes=nodes)
+        p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0} if nodes else None)
+        fine = GsProgram.of([(m, p)], np.float32, tile=64, defer=True)
+        coarse = GsProgram.of([(m, p)], np.float32, tile=256, defer=True)
+        D0, _ = coarse.relocatable_records()
+        base = 17  # the request's first descriptor slot in the server's table
+        req = SimpleNamespace(desc_base=base, n_w=np.asarray(D0["n_w"], dtype=np.int64))
+        assert fine.n_steps == coarse.n_steps
+        for tf, tc in zip(fine.tasks_host, coarse.tasks_host):
+            want = {(int(d) + base, int(s_), int(w)) for d, s_, w in np.asarray(tf)} if len(tf) else set()
+            got = set()
+            if len(tc):
+                rec = np.zeros(len(tc), dtype=TASK_DTYPE)
+                rec["desc"], rec["sd"], rec["w0"] = tc[:, 0] + base, tc[:, 1], tc[:, 2]
+                out = SolveServer._split(SolveServer, req, rec)
+                got = {(int(a["desc"]), int(a["sd"]), int(a["w0"])) for a in out}
+            assert got == want
