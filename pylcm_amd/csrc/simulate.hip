@@ -11,8 +11,13 @@ namespace lcm {
 
 constexpr int kSimBlock = 256;
 constexpr int kSimChunk = 128;  // actions staged in LDS at a time
-constexpr int kSimLanes = 8;    // lanes per agent; they split the agent's actions
+constexpr int kSimLanes = 4;    // lanes per agent; they split the agent's actions
 constexpr int kSimAgents = kSimBlock / kSimLanes;  // agents per workgroup
+
+// Loop over the agent features unrolled to the static maximum, so that phi[k]
+// stays a register (with the runtime bound nf each phi[k] becomes a chain of
+// selects).
+#define LCM_FOR_FEAT(k) _Pragma("unroll") for (int k = 0; k < LCM_SIM_MAX_FEAT; ++k) if (k < nf)
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
@@ -341,11 +346,13 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
     int nonfinite = 0;
     if (interp) {
       const T* EVg = static_cast<const T*>(D->EV);
-      for (int i = threadIdx.x; i < n_rows * g_n; i += kSimBlock) {
-        const int rr = i / g_n, k = i % g_n;
-        const T v = EVg[(int64_t(rr) * n_d + d) * g_n + k];
-        ev[i] = v;
-        nonfinite |= !is_finite(v);
+      for (int rr = 0; rr < n_rows; ++rr) {
+        const T* src = EVg + (int64_t(rr) * n_d + d) * g_n;
+        for (int k = threadIdx.x; k < g_n; k += kSimBlock) {
+          const T v = src[k];
+          ev[rr * g_n + k] = v;
+          nonfinite |= !is_finite(v);
+        }
       }
     }
     const bool finite = __syncthreads_or(nonfinite) == 0;
@@ -358,8 +365,8 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
       const int cn = min(kSimChunk, n_c - c0);
       __syncthreads();
       // stage tables for all sdd: tab[((s*ntab + j)*kSimChunk) + i]
-      for (int idx = threadIdx.x; idx < n_sdd * cn; idx += kSimBlock) {
-        const int s = idx / cn, i = idx % cn;
+      for (int s = 0; s < n_sdd; ++s)
+      for (int i = threadIdx.x; i < cn; i += kSimBlock) {
         const int64_t e = (int64_t(s) * n_d + d) * n_c + c0 + i;
         T* base = tab + int64_t(s) * ntab * kSimChunk + i;
         base[0 * kSimChunk] = static_cast<const T*>(D->U0)[e];
@@ -378,30 +385,87 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
       }
       __syncthreads();
       const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
-      for (int i = lane; i < cn; i += kSimLanes) {
+      int i0 = lane;
+      if (interp && lin && finite && !lo_feat && !hi_feat && !D->terminal && nf <= 1) {
+        // The common case (linear next grid, finite EV rows, interval bounds
+        // without features, at most one feature) four actions at a time, so
+        // that their LDS reads are in flight together; the same arithmetic and
+        // update order as the loop below, which takes the remaining actions.
+        const T* rl = ev + int64_t(row_lo) * g_n;
+        const T* rh = ev + int64_t(row_hi) * g_n;
+        const bool two = n_nodes > 1;
+        const T top = T(g_n - 2);
+        const T phi0 = phi[0];
+        auto quad = [&](auto nf_tag) {
+          constexpr int NF = decltype(nf_tag)::value;
+          auto q_at = [&](int i) -> T {
+            T u = mine[i];
+            if constexpr (NF == 1) u += mine[7 * kSimChunk + i] * phi0;
+            T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
+            if constexpr (NF == 1) x += mine[8 * kSimChunk + i] * phi0;
+            T coord;
+            if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
+            else coord = (x - g_start) * g_inv;
+            const T fl = fmin(fmax(floor(coord), T(0)), top);
+            const int li = int(fl);
+            const T t = coord - fl;
+            T v = (T(1) - tz) * ((T(1) - t) * rl[li] + t * rl[li + 1]);
+            if (two) v += tz * ((T(1) - t) * rh[li] + t * rh[li + 1]);
+            return u + beta * (cs + v);
+          };
+          auto take = [&](T q, bool feas, int i) {
+            if (feas) {
+              const bool isn = q != q;
+              nanf |= isn;
+              if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
+              if (q == q && (bidx < 0 || q > best)) {
+                best = q;
+                bidx = d * n_c + c0 + i;
+              }
+            }
+          };
+          for (; i0 + 3 * kSimLanes < cn; i0 += 4 * kSimLanes) {
+            bool f[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int i = i0 + k * kSimLanes;
+              f[k] = valid && (w >= mine[3 * kSimChunk + i]) && (w <= mine[4 * kSimChunk + i]);
+            }
+            if (__ballot(f[0] | f[1] | f[2] | f[3]) == 0) continue;
+            T q[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[k] = q_at(i0 + k * kSimLanes);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) take(q[k], f[k], i0 + k * kSimLanes);
+          }
+        };
+        if (nf == 1) quad(std::integral_constant<int, 1>{});
+        else quad(std::integral_constant<int, 0>{});
+      }
+      for (int i = i0; i < cn; i += kSimLanes) {
         T lo = mine[3 * kSimChunk + i], hi = mine[4 * kSimChunk + i];
         if (lo_feat) {
           T l = mine[5 * kSimChunk + i];
-          for (int k = 0; k < nf; ++k) l += mine[(7 + 2 * nf + k) * kSimChunk + i] * phi[k];
+          LCM_FOR_FEAT(k) l += mine[(7 + 2 * nf + k) * kSimChunk + i] * phi[k];
           if (D->lo_strict) l = nextafter(l, T(INFINITY));
           lo = fmax(lo, l);
         }
         if (hi_feat) {
           T h = mine[6 * kSimChunk + i];
-          for (int k = 0; k < nf; ++k) h += mine[(7 + 3 * nf + k) * kSimChunk + i] * phi[k];
+          LCM_FOR_FEAT(k) h += mine[(7 + 3 * nf + k) * kSimChunk + i] * phi[k];
           if (D->hi_strict) h = nextafter(h, T(-INFINITY));
           hi = fmin(hi, h);
         }
         const bool feas = valid && (w >= lo) && (w <= hi);
         if (__ballot(feas) == 0) continue;
         T u = mine[i];
-        for (int k = 0; k < nf; ++k) u += mine[(7 + k) * kSimChunk + i] * phi[k];
+        LCM_FOR_FEAT(k) u += mine[(7 + k) * kSimChunk + i] * phi[k];
         T q;
         if (D->terminal) {
           q = u;
         } else if (interp) {
           T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
-          for (int k = 0; k < nf; ++k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
+          LCM_FOR_FEAT(k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
           T coord;
           if (lin) {
             // linear grid: the same coordinate as the grid-search kernel
@@ -499,7 +563,7 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   const int d = a / n_c, c = a % n_c;
   const int64_t e = (int64_t(sdd) * n_d + d) * n_c + c;
   T x = static_cast<const T*>(D->A)[e] * w + static_cast<const T*>(D->B0)[e];
-  for (int k = 0; k < nf; ++k) x += (D->Bf[k] ? static_cast<const T*>(D->Bf[k])[e] : T(0)) * phi[k];
+  LCM_FOR_FEAT(k) x += (D->Bf[k] ? static_cast<const T*>(D->Bf[k])[e] : T(0)) * phi[k];
 
   const U4 rr = philox4x32_10(U4{uint32_t(agent), uint32_t(period), kStreamRegime, 0u}, k0, k1);
   const T* pr = static_cast<const T*>(D->preg) + (int64_t(sdd) * n_d + d) * D->n_regimes;
