@@ -271,9 +271,12 @@ int lcm_ev_launch(lcm_ctx* ctx, int dtype, const LcmEvDesc* d_descs, int n_desc,
  * is <= period take the argmax over the action grid; their next state and
  * next regime are drawn with Philox4x32-10 keyed by seed, counter
  * (agent, period, stream, 0). cur/next are the agent panels of this and the
- * next period; out receives this period's record. d_counts[n_regimes] and
- * d_order[n_regimes * n_agents] are device scratch for the partition of the
- * agents by regime. Replaces the period loop body of
+ * next period; out receives this period's record. d_counts[2 * n_regimes]
+ * and d_order[n_regimes * n_agents] are device scratch for the partition of
+ * the agents by regime: period p counts in half p % 2 and clears the other
+ * half for period p + 1, and the call for period 0 clears both, so periods
+ * are simulated in order from 0 with the same d_counts. Replaces the period
+ * loop body of
  * src/_lcm/simulation/simulate.py:296-378.
  */
 int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,

@@ -530,7 +530,7 @@ class SimProgram:
                              regime=alloc(n * 4), start=alloc(n * 4))
         off["out"] = dict(regime=alloc(T * n * 4), action=alloc(T * n * 4), value=alloc(T * n * isz),
                           w=alloc(T * n * isz), z=alloc(T * n * isz), sdd=alloc(T * n * 4))
-        off["counts"] = alloc(R * 4)
+        off["counts"] = alloc(2 * R * 4)  # two halves, alternating by period
         off["order"] = alloc(R * n * 4)
         # per-period tables, EV buffers and descriptors go into a second arena
         self.buf = nat.DeviceBuffer(sum(_align(p) for p in parts))

@@ -126,7 +126,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 partition_kernel(LcmAgents cur, LcmAgents nxt, int32_t* __restrict__ action_out,
                  int n, int period, int n_regimes, int32_t* __restrict__ counts,
-                 int32_t* __restrict__ order) {
+                 int32_t* __restrict__ next_counts, int32_t* __restrict__ order) {
   __shared__ int lcount[LCM_SIM_MAX_REGIMES];
   __shared__ int lbase[LCM_SIM_MAX_REGIMES];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -141,6 +141,9 @@ partition_kernel(LcmAgents cur, LcmAgents nxt, int32_t* __restrict__ action_out,
     if (cur.start[i] > period || r < 0 || r >= n_regimes) r = -1;
   }
   if (threadIdx.x < LCM_SIM_MAX_REGIMES) lcount[threadIdx.x] = 0;
+  // the other half of the counters serves the next period; the kernels that
+  // read it last (the previous period's) have finished
+  if (blockIdx.x == 0 && threadIdx.x < n_regimes) next_counts[threadIdx.x] = 0;
   __syncthreads();
   const int local = r >= 0 ? atomicAdd(&lcount[r], 1) : 0;
   __syncthreads();
@@ -647,15 +650,19 @@ extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_si
   if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_sim_period: LDS request above 160 KiB");
   const int nb = (n_agents + 255) / 256;
   const int nb_sim = (n_agents + kSimAgents - 1) / kSimAgents;
-  hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, ctx->stream, d_counts, n_regimes);
+  // counters: half period % 2 for this period; this period's partition clears
+  // the other half for the next one, so only the first period clears both
+  if (period == 0) hipLaunchKernelGGL(clear_kernel, dim3(1), dim3(64), 0, ctx->stream, d_counts, 2 * n_regimes);
+  int32_t* counts = d_counts + (period & 1) * n_regimes;
+  int32_t* next_counts = d_counts + ((period + 1) & 1) * n_regimes;
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   int32_t* action_out = out->action + int64_t(period) * n_agents;
   if (sz == 8)
     hipLaunchKernelGGL(partition_kernel<double>, dim3(nb), dim3(256), 0, ctx->stream, *cur, *next, action_out,
-                       n_agents, period, n_regimes, d_counts, d_order);
+                       n_agents, period, n_regimes, counts, next_counts, d_order);
   else
     hipLaunchKernelGGL(partition_kernel<float>, dim3(nb), dim3(256), 0, ctx->stream, *cur, *next, action_out,
-                       n_agents, period, n_regimes, d_counts, d_order);
+                       n_agents, period, n_regimes, counts, next_counts, d_order);
   dim3 grid(nb_sim, n_regimes);
   const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
   if (dtype == LCM_DTYPE_F32) {
@@ -663,13 +670,13 @@ extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_si
       LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(sim_kernel<float>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes)));
     hipLaunchKernelGGL(sim_kernel<float>, grid, dim3(kSimBlock), lds_bytes, ctx->stream, d_sim_descs,
-                       n_regimes, *cur, *next, *out, d_counts, d_order, n_agents, period, k0, k1);
+                       n_regimes, *cur, *next, *out, counts, d_order, n_agents, period, k0, k1);
   } else if (dtype == LCM_DTYPE_F64) {
     if (lds_bytes > 64 * 1024)
       LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(sim_kernel<double>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes)));
     hipLaunchKernelGGL(sim_kernel<double>, grid, dim3(kSimBlock), lds_bytes, ctx->stream, d_sim_descs,
-                       n_regimes, *cur, *next, *out, d_counts, d_order, n_agents, period, k0, k1);
+                       n_regimes, *cur, *next, *out, counts, d_order, n_agents, period, k0, k1);
   } else {
     return fail(LCM_E_ARG, "lcm_sim_period: unknown dtype");
   }
