@@ -172,6 +172,9 @@ typedef struct LcmSimDesc {
    * Qc_d the masked max over the continuous actions
    * (src/_lcm/regime_building/max_Q_over_a.py:255-346) */
   double taste_scale;
+  /* 1: the EV rows of this regime are read from global memory (they do not
+   * fit in LDS next to the action tables); 0: staged in LDS per cell */
+  int32_t ev_global, pad_;
 } LcmSimDesc;
 
 /* Agent panel state (device arrays of n agents). */

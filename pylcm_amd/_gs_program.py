@@ -28,6 +28,7 @@ from pylcm_amd.exceptions import UnsupportedModelError
 ALIGN = 256
 DESC_DTYPE = np.dtype(nat.LcmGsDesc)
 TASK_DTYPE = np.dtype(nat.LcmGsTask)
+LDS_LIMIT = 160 * 1024  # bytes of LDS one workgroup may use on gfx950
 FLOPS_PER_Q = 10  # budget 2, coordinate 2, interpolation 3, aggregator 2, max 1
 
 
@@ -181,7 +182,14 @@ def step_lds_bytes(dtype, gmax, kmax):
     """`lcm_gs_lds_bytes` for each step's largest next grid and constraint count."""
     code = nat.LCM_DTYPE_F64 if np.dtype(dtype) == np.float64 else nat.LCM_DTYPE_F32
     L = nat.load_library()
-    return [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
+    out = [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
+    for b, g, k in zip(out, gmax, kmax):
+        if b > LDS_LIMIT:
+            raise UnsupportedModelError(
+                f"a next-state grid of {int(g)} points ({int(k)} general constraints) needs {b} B of LDS "
+                f"per workgroup in {np.dtype(dtype)}, above the {LDS_LIMIT} B of a gfx950 compute unit; "
+                "use a coarser grid or float32")
+    return out
 
 
 class GsProgram:

@@ -96,6 +96,7 @@ class LcmSimDesc(C.Structure):
         ("beta", C.c_double), ("g_start", C.c_double), ("g_stop", C.c_double),
         ("pa", C.c_double), ("pb", C.c_double), ("pc", C.c_double),
         ("taste_scale", C.c_double),
+        ("ev_global", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

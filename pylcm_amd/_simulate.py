@@ -282,6 +282,10 @@ class SimLowering(PeriodLowering):
 # ---------------------------------------------------------------------------
 
 
+# LDS one simulation workgroup may use (gfx950: 160 KiB per workgroup)
+SIM_LDS_LIMIT = 160 * 1024
+
+
 class SimulationResult:
     """Simulated panel (`src/lcm/result.py:125-185`)."""
 
@@ -593,9 +597,16 @@ class SimProgram:
                 ntab = st.n_sdd * (7 + 4 * len(st.feats))
                 L = nat.load_library()
                 code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
-                lds_max = max(lds_max, int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab)))
-        if lds_max > 160 * 1024:
-            raise UnsupportedModelError("simulation tables do not fit in LDS (next-state grid too large)")
+                need = int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab))
+                # EV rows that do not fit in LDS next to the tables are read
+                # from global memory
+                sim_desc[(t, r)]["ev_global"] = int(need > SIM_LDS_LIMIT)
+                if need > SIM_LDS_LIMIT:
+                    need = int(L.lcm_sim_lds_bytes(code, g_n, 0, ntab))
+                lds_max = max(lds_max, need)
+        if lds_max > SIM_LDS_LIMIT:
+            raise UnsupportedModelError("simulation action tables and next-state grid do not fit in LDS "
+                                        f"({lds_max} B > {SIM_LDS_LIMIT} B)")
         # EV descriptors point at the solve program's own descriptors
         gs_index = self._gs_desc_index()
         ev_descs = np.zeros(len(ev_meta), dtype=np.dtype(nat.LcmEvDesc))

@@ -319,7 +319,9 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
 
   // LDS: EV rows of the current d [n_rows][g_n], grid points, action chunk tables
   T* ev = reinterpret_cast<T*>(smem);
-  T* gp = ev + (n_rows * g_n > 0 ? n_rows * g_n : 1);
+  // EV rows in LDS, or read in place from global memory when they do not fit
+  const bool ev_in_lds = D->ev_global == 0;
+  T* gp = ev + (ev_in_lds && n_rows * g_n > 0 ? n_rows * g_n : 1);
   // staged tables per action: U0 A B0 lo hi LoB HiB Uf.. Bf.. LoF.. HiF..
   const int ntab = 7 + 4 * nf;
   const bool lo_feat = D->LoB != nullptr, hi_feat = D->HiB != nullptr;
@@ -353,7 +355,7 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
         const T* src = EVg + (int64_t(rr) * n_d + d) * g_n;
         for (int k = threadIdx.x; k < g_n; k += kSimBlock) {
           const T v = src[k];
-          ev[rr * g_n + k] = v;
+          if (ev_in_lds) ev[rr * g_n + k] = v;
           nonfinite |= !is_finite(v);
         }
       }
@@ -387,118 +389,129 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
         }
       }
       __syncthreads();
-      const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
-      int i0 = lane;
-      if (interp && lin && finite && !lo_feat && !hi_feat && !D->terminal && nf <= 1) {
-        // The common case (linear next grid, finite EV rows, interval bounds
-        // without features, at most one feature) four actions at a time, so
-        // that their LDS reads are in flight together; the same arithmetic and
-        // update order as the loop below, which takes the remaining actions.
-        const T* rl = ev + int64_t(row_lo) * g_n;
-        const T* rh = ev + int64_t(row_hi) * g_n;
-        const bool two = n_nodes > 1;
-        const T top = T(g_n - 2);
-        const T phi0 = phi[0];
-        auto quad = [&](auto nf_tag) {
-          constexpr int NF = decltype(nf_tag)::value;
-          auto q_at = [&](int i) -> T {
-            T u = mine[i];
-            if constexpr (NF == 1) u += mine[7 * kSimChunk + i] * phi0;
-            T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
-            if constexpr (NF == 1) x += mine[8 * kSimChunk + i] * phi0;
-            T coord;
-            if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
-            else coord = (x - g_start) * g_inv;
-            const T fl = fmin(fmax(floor(coord), T(0)), top);
-            const int li = int(fl);
-            const T t = coord - fl;
-            T v = (T(1) - tz) * ((T(1) - t) * rl[li] + t * rl[li + 1]);
-            if (two) v += tz * ((T(1) - t) * rh[li] + t * rh[li + 1]);
-            return u + beta * (cs + v);
-          };
-          auto take = [&](T q, bool feas, int i) {
-            if (feas) {
-              const bool isn = q != q;
-              nanf |= isn;
-              if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
-              if (q == q && (bidx < 0 || q > best)) {
-                best = q;
-                bidx = d * n_c + c0 + i;
+      // the action loops, instantiated for EV rows in LDS and in global memory
+      // (one address space per instantiation keeps the gathers ds_read / global loads)
+      auto actions = [&](auto lds_tag) {
+        constexpr bool IN_LDS = decltype(lds_tag)::value;
+        const T* evb;
+        if constexpr (IN_LDS) evb = ev;
+        else evb = static_cast<const T*>(D->EV) + int64_t(d) * g_n;
+        const int64_t ev_rs = IN_LDS ? int64_t(g_n) : int64_t(n_d) * g_n;
+        const T* mine = tab + int64_t(sdd) * ntab * kSimChunk;
+        int i0 = lane;
+        if (interp && lin && finite && !lo_feat && !hi_feat && !D->terminal && nf <= 1) {
+          // The common case (linear next grid, finite EV rows, interval bounds
+          // without features, at most one feature) four actions at a time, so
+          // that their LDS reads are in flight together; the same arithmetic and
+          // update order as the loop below, which takes the remaining actions.
+          const T* rl = evb + int64_t(row_lo) * ev_rs;
+          const T* rh = evb + int64_t(row_hi) * ev_rs;
+          const bool two = n_nodes > 1;
+          const T top = T(g_n - 2);
+          const T phi0 = phi[0];
+          auto quad = [&](auto nf_tag) {
+            constexpr int NF = decltype(nf_tag)::value;
+            auto q_at = [&](int i) -> T {
+              T u = mine[i];
+              if constexpr (NF == 1) u += mine[7 * kSimChunk + i] * phi0;
+              T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
+              if constexpr (NF == 1) x += mine[8 * kSimChunk + i] * phi0;
+              T coord;
+              if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
+              else coord = (x - g_start) * g_inv;
+              const T fl = fmin(fmax(floor(coord), T(0)), top);
+              const int li = int(fl);
+              const T t = coord - fl;
+              T v = (T(1) - tz) * ((T(1) - t) * rl[li] + t * rl[li + 1]);
+              if (two) v += tz * ((T(1) - t) * rh[li] + t * rh[li + 1]);
+              return u + beta * (cs + v);
+            };
+            auto take = [&](T q, bool feas, int i) {
+              if (feas) {
+                const bool isn = q != q;
+                nanf |= isn;
+                if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
+                if (q == q && (bidx < 0 || q > best)) {
+                  best = q;
+                  bidx = d * n_c + c0 + i;
+                }
               }
+            };
+            for (; i0 + 3 * kSimLanes < cn; i0 += 4 * kSimLanes) {
+              bool f[4];
+  #pragma unroll
+              for (int k = 0; k < 4; ++k) {
+                const int i = i0 + k * kSimLanes;
+                f[k] = valid && (w >= mine[3 * kSimChunk + i]) && (w <= mine[4 * kSimChunk + i]);
+              }
+              if (__ballot(f[0] | f[1] | f[2] | f[3]) == 0) continue;
+              T q[4];
+  #pragma unroll
+              for (int k = 0; k < 4; ++k) q[k] = q_at(i0 + k * kSimLanes);
+  #pragma unroll
+              for (int k = 0; k < 4; ++k) take(q[k], f[k], i0 + k * kSimLanes);
             }
           };
-          for (; i0 + 3 * kSimLanes < cn; i0 += 4 * kSimLanes) {
-            bool f[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const int i = i0 + k * kSimLanes;
-              f[k] = valid && (w >= mine[3 * kSimChunk + i]) && (w <= mine[4 * kSimChunk + i]);
+          if (nf == 1) quad(std::integral_constant<int, 1>{});
+          else quad(std::integral_constant<int, 0>{});
+        }
+        for (int i = i0; i < cn; i += kSimLanes) {
+          T lo = mine[3 * kSimChunk + i], hi = mine[4 * kSimChunk + i];
+          if (lo_feat) {
+            T l = mine[5 * kSimChunk + i];
+            LCM_FOR_FEAT(k) l += mine[(7 + 2 * nf + k) * kSimChunk + i] * phi[k];
+            if (D->lo_strict) l = nextafter(l, T(INFINITY));
+            lo = fmax(lo, l);
+          }
+          if (hi_feat) {
+            T h = mine[6 * kSimChunk + i];
+            LCM_FOR_FEAT(k) h += mine[(7 + 3 * nf + k) * kSimChunk + i] * phi[k];
+            if (D->hi_strict) h = nextafter(h, T(-INFINITY));
+            hi = fmin(hi, h);
+          }
+          const bool feas = valid && (w >= lo) && (w <= hi);
+          if (__ballot(feas) == 0) continue;
+          T u = mine[i];
+          LCM_FOR_FEAT(k) u += mine[(7 + k) * kSimChunk + i] * phi[k];
+          T q;
+          if (D->terminal) {
+            q = u;
+          } else if (interp) {
+            T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
+            LCM_FOR_FEAT(k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
+            T coord;
+            if (lin) {
+              // linear grid: the same coordinate as the grid-search kernel
+              if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
+              else coord = (x - g_start) * g_inv;
+            } else {
+              coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
             }
-            if (__ballot(f[0] | f[1] | f[2] | f[3]) == 0) continue;
-            T q[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) q[k] = q_at(i0 + k * kSimLanes);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) take(q[k], f[k], i0 + k * kSimLanes);
-          }
-        };
-        if (nf == 1) quad(std::integral_constant<int, 1>{});
-        else quad(std::integral_constant<int, 0>{});
-      }
-      for (int i = i0; i < cn; i += kSimLanes) {
-        T lo = mine[3 * kSimChunk + i], hi = mine[4 * kSimChunk + i];
-        if (lo_feat) {
-          T l = mine[5 * kSimChunk + i];
-          LCM_FOR_FEAT(k) l += mine[(7 + 2 * nf + k) * kSimChunk + i] * phi[k];
-          if (D->lo_strict) l = nextafter(l, T(INFINITY));
-          lo = fmax(lo, l);
-        }
-        if (hi_feat) {
-          T h = mine[6 * kSimChunk + i];
-          LCM_FOR_FEAT(k) h += mine[(7 + 3 * nf + k) * kSimChunk + i] * phi[k];
-          if (D->hi_strict) h = nextafter(h, T(-INFINITY));
-          hi = fmin(hi, h);
-        }
-        const bool feas = valid && (w >= lo) && (w <= hi);
-        if (__ballot(feas) == 0) continue;
-        T u = mine[i];
-        LCM_FOR_FEAT(k) u += mine[(7 + k) * kSimChunk + i] * phi[k];
-        T q;
-        if (D->terminal) {
-          q = u;
-        } else if (interp) {
-          T x = mine[1 * kSimChunk + i] * w + mine[2 * kSimChunk + i];
-          LCM_FOR_FEAT(k) x += mine[(7 + nf + k) * kSimChunk + i] * phi[k];
-          T coord;
-          if (lin) {
-            // linear grid: the same coordinate as the grid-search kernel
-            if constexpr (std::is_same<T, double>::value) coord = (x - g_start) / g_step;
-            else coord = (x - g_start) * g_inv;
+            T v;
+            if (finite) {
+              v = (T(1) - tz) * interp_at<T, true>(evb + int64_t(row_lo) * ev_rs, g_n, coord);
+              if (n_nodes > 1) v += tz * interp_at<T, true>(evb + int64_t(row_hi) * ev_rs, g_n, coord);
+            } else {
+              v = zero_safe(T(1) - tz, interp_at<T, false>(evb + int64_t(row_lo) * ev_rs, g_n, coord));
+              if (n_nodes > 1) v += zero_safe(tz, interp_at<T, false>(evb + int64_t(row_hi) * ev_rs, g_n, coord));
+            }
+            q = u + beta * (cs + v);
           } else {
-            coord = sim_coordinate<T>(D->g_kind, g_n, g_start, g_stop, gp, x);
+            q = u + beta * cs;
           }
-          T v;
-          if (finite) {
-            v = (T(1) - tz) * interp_at<T, true>(ev + int64_t(row_lo) * g_n, g_n, coord);
-            if (n_nodes > 1) v += tz * interp_at<T, true>(ev + int64_t(row_hi) * g_n, g_n, coord);
-          } else {
-            v = zero_safe(T(1) - tz, interp_at<T, false>(ev + int64_t(row_lo) * g_n, g_n, coord));
-            if (n_nodes > 1) v += zero_safe(tz, interp_at<T, false>(ev + int64_t(row_hi) * g_n, g_n, coord));
-          }
-          q = u + beta * (cs + v);
-        } else {
-          q = u + beta * cs;
-        }
-        if (feas) {
-          const bool isn = q != q;
-          nanf |= isn;
-          if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
-          if (q == q && (bidx < 0 || q > best)) {
-            best = q;
-            bidx = d * n_c + c0 + i;
+          if (feas) {
+            const bool isn = q != q;
+            nanf |= isn;
+            if (isn && nan_idx == INT_MAX) nan_idx = d * n_c + c0 + i;
+            if (q == q && (bidx < 0 || q > best)) {
+              best = q;
+              bidx = d * n_c + c0 + i;
+            }
           }
         }
-      }
+      };
+      if (ev_in_lds) actions(std::true_type{});
+      else actions(std::false_type{});
     }
     if (taste > 0.0) {
       const SimBest<T> mb = sim_merge_lanes(SimBest<T>{best, bidx, nanf});

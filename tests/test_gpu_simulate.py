@@ -104,6 +104,24 @@ def test_simulate_processes_model_f64(native, dist):
     _compare(m, p, ic, np.float64, seed=9, rtol=1e-9)
 
 
+def test_simulate_ev_rows_from_global_memory_f64(native):
+    """7 income nodes x 3000 wealth points in float64: the EV rows (168 KB) do
+    not fit in LDS, so the kernel reads them in place from global memory; the
+    paths equal the oracle's."""
+    from pylcm_amd import _native as nat
+    from pylcm_amd._simulate import SIM_LDS_LIMIT
+
+    T = 4
+    m = ex.retirement_model(T, n_wealth=3000, n_consumption=60, income_nodes=7)
+    p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.2, "mu": 0.0})
+    assert nat.load_library().lcm_sim_lds_bytes(nat.LCM_DTYPE_F64, 3000, 7, 11) > SIM_LDS_LIMIT
+    rng = np.random.default_rng(4)
+    n = 150
+    ic = {"wealth": rng.uniform(1, 400, n), "income": rng.normal(0, 0.3, n),
+          "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    _compare(m, p, ic, np.float64, seed=9, rtol=1e-9)
+
+
 def test_simulate_large_panel_f32(native):
     """100k agents of the stochastic model: runs, first-period value equals the
     solve's V where the agent starts on the grid, income moments follow AR(1)."""

@@ -134,3 +134,23 @@ def test_split_tasks_are_the_64_state_tiling():
                 out = SolveServer._split(SolveServer, req, rec)
                 got = {(int(a["desc"]), int(a["sd"]), int(a["w0"])) for a in out}
             assert got == want
+
+
+def test_grid_above_lds_capacity_is_a_clear_error():
+    """A next-state grid whose EV row does not fit the workgroup's LDS is
+    refused with UnsupportedModelError before anything is launched."""
+    import numpy as np
+    import pytest
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._gs_program import GsProgram
+    from pylcm_amd.exceptions import UnsupportedModelError
+
+    m = ex.retirement_model(3, n_wealth=12000, n_consumption=20)
+    p = ex.retirement_params(3)
+    prog = GsProgram.of([(m, p)], np.float64, tile=256, defer=True)
+    with pytest.raises(UnsupportedModelError, match="LDS"):
+        prog.lds_by_step
+    ok = GsProgram.of([(ex.retirement_model(3, n_wealth=2000, n_consumption=20), p)], np.float64,
+                      tile=256, defer=True)
+    assert max(ok.lds_by_step) <= 160 * 1024
