@@ -377,7 +377,7 @@ def main():
     ap.add_argument("--draws", type=int, default=1024)
     ap.add_argument("--periods", type=int, default=40)
     ap.add_argument("--plain-sample", type=int, default=48)
-    ap.add_argument("--cpu-sample", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--copy-calib-mib", type=int, default=256)
@@ -386,7 +386,7 @@ def main():
     ap.add_argument("--dcegm-batch", type=int, default=64)
     ap.add_argument("--no-sim", action="store_true")
     ap.add_argument("--sim-agents", type=int, default=100_000)
-    ap.add_argument("--sim-cpu-agents", type=int, default=500)
+    ap.add_argument("--sim-cpu-agents", type=int, default=1500)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-sat-rate", type=float, default=1000.0)

@@ -13,6 +13,7 @@ constexpr int kSimBlock = 256;
 constexpr int kSimChunk = 128;  // actions staged in LDS at a time
 constexpr int kSimLanes = 4;    // lanes per agent; they split the agent's actions
 constexpr int kSimAgents = kSimBlock / kSimLanes;  // agents per workgroup
+constexpr int kPartAgents = 4;  // agents per thread of the partition kernel
 
 // Loop over the agent features unrolled to the static maximum, so that phi[k]
 // stays a register (with the runtime bound nf each phi[k] becomes a chain of
@@ -129,28 +130,39 @@ partition_kernel(LcmAgents cur, LcmAgents nxt, int32_t* __restrict__ action_out,
                  int32_t* __restrict__ next_counts, int32_t* __restrict__ order) {
   __shared__ int lcount[LCM_SIM_MAX_REGIMES];
   __shared__ int lbase[LCM_SIM_MAX_REGIMES];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int r = -1;
-  if (i < n) {
-    r = cur.regime[i];
-    nxt.regime[i] = r;
-    nxt.sdd[i] = cur.sdd[i];
-    static_cast<T*>(nxt.w)[i] = static_cast<const T*>(cur.w)[i];
-    static_cast<T*>(nxt.z)[i] = static_cast<const T*>(cur.z)[i];
-    action_out[i] = -1;
-    if (cur.start[i] > period || r < 0 || r >= n_regimes) r = -1;
-  }
+  // kPartAgents agents per thread (strided, so every access stays coalesced):
+  // fewer workgroups, so fewer global atomics on the per-regime counters
+  int r[kPartAgents], local[kPartAgents];
   if (threadIdx.x < LCM_SIM_MAX_REGIMES) lcount[threadIdx.x] = 0;
   // the other half of the counters serves the next period; the kernels that
   // read it last (the previous period's) have finished
   if (blockIdx.x == 0 && threadIdx.x < n_regimes) next_counts[threadIdx.x] = 0;
   __syncthreads();
-  const int local = r >= 0 ? atomicAdd(&lcount[r], 1) : 0;
+#pragma unroll
+  for (int k = 0; k < kPartAgents; ++k) {
+    const int i = (blockIdx.x * kPartAgents + k) * blockDim.x + threadIdx.x;
+    r[k] = -1;
+    if (i < n) {
+      int rr = cur.regime[i];
+      nxt.regime[i] = rr;
+      nxt.sdd[i] = cur.sdd[i];
+      static_cast<T*>(nxt.w)[i] = static_cast<const T*>(cur.w)[i];
+      static_cast<T*>(nxt.z)[i] = static_cast<const T*>(cur.z)[i];
+      action_out[i] = -1;
+      if (cur.start[i] > period || rr < 0 || rr >= n_regimes) rr = -1;
+      r[k] = rr;
+    }
+    local[k] = r[k] >= 0 ? atomicAdd(&lcount[r[k]], 1) : 0;
+  }
   __syncthreads();
   if (threadIdx.x < n_regimes && lcount[threadIdx.x] > 0)
     lbase[threadIdx.x] = atomicAdd(&counts[threadIdx.x], lcount[threadIdx.x]);
   __syncthreads();
-  if (r >= 0) order[int64_t(r) * n + lbase[r] + local] = i;
+#pragma unroll
+  for (int k = 0; k < kPartAgents; ++k) {
+    const int i = (blockIdx.x * kPartAgents + k) * blockDim.x + threadIdx.x;
+    if (r[k] >= 0) order[int64_t(r[k]) * n + lbase[r[k]] + local[k]] = i;
+  }
 }
 
 __global__ void clear_kernel(int32_t* p, int n) {
@@ -661,7 +673,7 @@ extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_si
   if (n_agents <= 0 || n_regimes <= 0 || n_regimes > LCM_SIM_MAX_REGIMES)
     return fail(LCM_E_ARG, "lcm_sim_period: bad sizes");
   if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_sim_period: LDS request above 160 KiB");
-  const int nb = (n_agents + 255) / 256;
+  const int nb = (n_agents + 256 * kPartAgents - 1) / (256 * kPartAgents);
   const int nb_sim = (n_agents + kSimAgents - 1) / kSimAgents;
   // counters: half period % 2 for this period; this period's partition clears
   // the other half for the next one, so only the first period clears both
