@@ -115,6 +115,7 @@ typedef struct LcmEvDesc {
   void* EV;
   void* CS;
   int32_t n_sd, n_d, g_n, pad;
+  int32_t* fin; /* per EV row: 1 if every entry is finite (written with the row) */
 } LcmEvDesc;
 
 #define LCM_SIM_MAX_FEAT 4
@@ -175,6 +176,9 @@ typedef struct LcmSimDesc {
   /* 1: the EV rows of this regime are read from global memory (they do not
    * fit in LDS next to the action tables); 0: staged in LDS per cell */
   int32_t ev_global, pad_;
+  /* per EV row (n_rows x n_d): 1 if finite, as written by lcm_ev_launch; read
+   * instead of scanning the rows when they stay in global memory */
+  const int32_t* ev_fin;
 } LcmSimDesc;
 
 /* Agent panel state (device arrays of n agents). */

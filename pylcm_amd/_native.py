@@ -74,7 +74,8 @@ SIM_MAX_REGIMES = 8
 
 class LcmEvDesc(C.Structure):
     _fields_ = [("gs", vp), ("EV", vp), ("CS", vp),
-                ("n_sd", C.c_int32), ("n_d", C.c_int32), ("g_n", C.c_int32), ("pad", C.c_int32)]
+                ("n_sd", C.c_int32), ("n_d", C.c_int32), ("g_n", C.c_int32), ("pad", C.c_int32),
+                ("fin", vp)]
 
 
 class LcmSimDesc(C.Structure):
@@ -97,6 +98,7 @@ class LcmSimDesc(C.Structure):
         ("pa", C.c_double), ("pb", C.c_double), ("pc", C.c_double),
         ("taste_scale", C.c_double),
         ("ev_global", C.c_int32), ("pad_", C.c_int32),
+        ("ev_fin", vp),
     ]
 
 

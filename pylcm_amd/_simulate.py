@@ -284,6 +284,8 @@ class SimLowering(PeriodLowering):
 
 # LDS one simulation workgroup may use (gfx950: 160 KiB per workgroup)
 SIM_LDS_LIMIT = 160 * 1024
+# EV rows of one cell up to this size are read from global memory (L1-resident)
+SIM_EV_L1_BYTES = 32 * 1024
 
 
 class SimulationResult:
@@ -582,7 +584,8 @@ class SimProgram:
             # g_n = 0: only stateless targets, the kernel fills CS alone
             g_n = len(self.rg.w_pts[pt.next_grid]) if pt.targets else 0
             ev_meta[(t, r)] = (ar.reserve(max(lay.n_sd * lay.n_d * g_n, 1) * isz),
-                               ar.reserve(lay.n_sd * lay.n_d * isz), g_n)
+                               ar.reserve(lay.n_sd * lay.n_d * isz), g_n,
+                               ar.reserve(lay.n_sd * lay.n_d * 4))  # per-row finite flags
         sim_desc = {}
         lds_max = 0
         keep_tables = []
@@ -593,15 +596,18 @@ class SimProgram:
                 st = self.low.lower_sim(sl, t)
                 keep_tables.append(st)
                 sim_desc[(t, r)] = self._desc(st, sl, t, r, ar, zero, ev_meta)
-                g_n = ev_meta.get((t, r), (0, 0, 0))[2]
+                g_n = ev_meta.get((t, r), (0, 0, 0, 0))[2]
                 ntab = st.n_sdd * (7 + 4 * len(st.feats))
                 L = nat.load_library()
                 code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
                 need = int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab))
-                # EV rows that do not fit in LDS next to the tables are read
-                # from global memory
-                sim_desc[(t, r)]["ev_global"] = int(need > SIM_LDS_LIMIT)
-                if need > SIM_LDS_LIMIT:
+                # EV rows are read in place from global memory when one cell's
+                # rows fit in a CU's L1 (no staging pass) or do not fit in LDS
+                # next to the tables; otherwise they are staged in LDS
+                cell_bytes = lay.n_sd * g_n * dt.itemsize
+                ev_global = need > SIM_LDS_LIMIT or cell_bytes <= SIM_EV_L1_BYTES
+                sim_desc[(t, r)]["ev_global"] = int(ev_global)
+                if ev_global:
                     need = int(L.lcm_sim_lds_bytes(code, g_n, 0, ntab))
                 lds_max = max(lds_max, need)
         if lds_max > SIM_LDS_LIMIT:
@@ -619,10 +625,11 @@ class SimProgram:
         for i, key in enumerate(ev_keys):
             t, r = key
             lay = model.layouts[r]
-            e_off, c_off, g_n = ev_meta[key]
+            e_off, c_off, g_n, f_off = ev_meta[key]
             ev_descs[i]["gs"] = prog.base + prog.desc_off + gs_index[key] * DESC_DTYPE.itemsize
             ev_descs[i]["EV"] = b2 + e_off
             ev_descs[i]["CS"] = b2 + c_off
+            ev_descs[i]["fin"] = b2 + f_off
             ev_descs[i]["n_sd"], ev_descs[i]["n_d"], ev_descs[i]["g_n"] = lay.n_sd, lay.n_d, g_n
         SD = np.zeros(T * R, dtype=np.dtype(nat.LcmSimDesc))
         for (t, r), d in sim_desc.items():
@@ -634,7 +641,7 @@ class SimProgram:
                 elif k in ("n_sdd_next", "feat_op"):
                     for j, p in enumerate(v):
                         rec[k][j] = p
-                elif k in ("U0", "A", "B0", "lo", "hi", "EV", "CS", "sd_base", "nodes", "g_pts", "preg",
+                elif k in ("U0", "A", "B0", "lo", "hi", "EV", "CS", "ev_fin", "sd_base", "nodes", "g_pts", "preg",
                            "LoB", "HiB"):
                     rec[k] = (b2 + v) if v is not None else 0
                 else:
@@ -746,7 +753,7 @@ class SimProgram:
         d["regime_code"] = model.regime_code(r)
         d["n_regimes"] = len(model.regime_names)
         d["n_rows"] = lay.n_sd
-        d["EV"] = d["CS"] = None
+        d["EV"] = d["CS"] = d["ev_fin"] = None
         d["sd_base"] = ar.add(sl.sd_base)
         d["nodes"] = zero
         d["g_pts"] = zero
@@ -780,8 +787,8 @@ class SimProgram:
                 code = model.regime_code(tr_name)
                 d["Wd"][code] = ar.add(W, dtype=dt)
                 d["n_sdd_next"][code] = W.shape[-1]
-            e_off, c_off, g_n = ev_meta[(t, r)]
-            d["EV"], d["CS"] = e_off, c_off
+            e_off, c_off, g_n, f_off = ev_meta[(t, r)]
+            d["EV"], d["CS"], d["ev_fin"] = e_off, c_off, f_off
             if g_n > 0:
                 d["interp"] = 1
                 glay = model.layouts[pt.next_grid]

@@ -90,6 +90,7 @@ ev_kernel(const LcmEvDesc* __restrict__ descs, int max_rows) {
   const int n_d = E.n_d, g_n = E.g_n;
   const int sd = row / n_d, d = row % n_d;
   T* out = static_cast<T*>(E.EV) + int64_t(row) * g_n;
+  int nonfinite = 0;
   for (int k = threadIdx.x; k < g_n; k += blockDim.x) {
     T acc = T(0);
     for (int r = 0; r < D->n_targets; ++r) {
@@ -102,7 +103,10 @@ ev_kernel(const LcmEvDesc* __restrict__ descs, int max_rows) {
       }
     }
     out[k] = acc;
+    nonfinite |= !is_finite(acc);
   }
+  const bool row_finite = __syncthreads_or(nonfinite) == 0;
+  if (threadIdx.x == 0 && E.fin) E.fin[row] = row_finite ? 1 : 0;
   if (threadIdx.x == 0) {
     T cs = T(0);
     for (int s = 0; s < D->n_scalar; ++s) {
@@ -361,7 +365,10 @@ sim_kernel(const LcmSimDesc* __restrict__ descs, int n_regimes, LcmAgents cur, L
   for (int d = 0; d < n_d; ++d) {
     __syncthreads();
     int nonfinite = 0;
-    if (interp) {
+    if (interp && !ev_in_lds && D->ev_fin) {
+      // rows read in place: their finiteness was recorded when they were written
+      for (int rr = 0; rr < n_rows; ++rr) nonfinite |= D->ev_fin[rr * n_d + d] == 0;
+    } else if (interp) {
       const T* EVg = static_cast<const T*>(D->EV);
       for (int rr = 0; rr < n_rows; ++rr) {
         const T* src = EVg + (int64_t(rr) * n_d + d) * g_n;

@@ -122,6 +122,23 @@ def test_simulate_ev_rows_from_global_memory_f64(native):
     _compare(m, p, ic, np.float64, seed=9, rtol=1e-9)
 
 
+def test_simulate_ev_rows_staged_in_lds_f64(native):
+    """7 income nodes x 1000 wealth points in float64: one cell's EV rows
+    (56 KB) exceed a CU's L1 but fit in LDS, so the kernel stages them; the
+    paths equal the oracle's. (Smaller cells are read in place.)"""
+    from pylcm_amd._simulate import SIM_EV_L1_BYTES, SIM_LDS_LIMIT
+
+    assert SIM_EV_L1_BYTES < 7 * 1000 * 8 < SIM_LDS_LIMIT
+    T = 4
+    m = ex.retirement_model(T, n_wealth=1000, n_consumption=60, income_nodes=7)
+    p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.2, "mu": 0.0})
+    rng = np.random.default_rng(6)
+    n = 150
+    ic = {"wealth": rng.uniform(1, 400, n), "income": rng.normal(0, 0.3, n),
+          "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    _compare(m, p, ic, np.float64, seed=10, rtol=1e-9)
+
+
 def test_simulate_large_panel_f32(native):
     """100k agents of the stochastic model: runs, first-period value equals the
     solve's V where the agent starts on the grid, income moments follow AR(1)."""
