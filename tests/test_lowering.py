@@ -128,3 +128,18 @@ def test_library_exports_every_declared_symbol():
     a, b = ctypes.c_size_t(), ctypes.c_size_t()
     lib.lcm_abi_sizes(ctypes.byref(a), ctypes.byref(b))
     assert a.value == ctypes.sizeof(_native.LcmGsDesc) and b.value == ctypes.sizeof(_native.LcmGsTask)
+
+
+def test_struct_layouts_match_the_library():
+    """Every ctypes mirror has the size the library was compiled with."""
+    from pylcm_amd import _native
+
+    lib = _native.load_library()
+    s = [ctypes.c_size_t() for _ in range(4)]
+    assert lib.lcm_abi_sizes_sim(*(ctypes.byref(x) for x in s)) == 0
+    assert [x.value for x in s] == [ctypes.sizeof(_native.LcmEvDesc), ctypes.sizeof(_native.LcmSimDesc),
+                                    ctypes.sizeof(_native.LcmAgents), ctypes.sizeof(_native.LcmSimOut)]
+    e = [ctypes.c_size_t() for _ in range(3)]
+    assert lib.lcm_abi_sizes_egm(*(ctypes.byref(x) for x in e)) == 0
+    assert [x.value for x in e] == [ctypes.sizeof(_native.LcmEgmRow), ctypes.sizeof(_native.LcmEgmCol),
+                                    ctypes.sizeof(_native.LcmEgmAgg)]
