@@ -369,21 +369,24 @@ __device__ __forceinline__ void aggregate_rows(const double* vals, const double*
   }
 }
 
-constexpr int kNodes = 64;  // savings nodes per block of the Euler kernel
+// Savings nodes per block of the Euler kernel: 64, 128 or 256 (at most one
+// per thread in the aggregation phase), chosen per launch by the grid size.
+constexpr int kNodesMin = 64, kNodesMax = 256;
 constexpr int kMarginalBlocks = 16;  // workgroups per column for the marginal row (4096 slots)
 constexpr int kPublishBlocks = 8;    // workgroups per column for the V row (2048 wealth nodes)
 
 // Phase 1: continuation and Euler inversion. Block (bx, col) takes savings
-// nodes [64 bx, 64 bx + 64) of column col; its 256 threads read every (child
-// row, node) pair in parallel into LDS, then one thread per node aggregates
+// nodes [NODES bx, NODES bx + NODES) of column col; its 256 threads read every
+// (child row, node) pair in parallel into LDS, then one thread per node aggregates
 // over the child's choice rows, nodes and target regimes and inverts the
 // Euler equation. Writes candidate slots n_con + j and, for node 0, E[V'] at
 // the savings limit (cand[4n]).
+template <int NODES>
 __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __restrict__ cols) {
   const LcmEgmCol& C = cols[blockIdx.y];
   const int tid = threadIdx.x;
   const int n_s = C.n_s, n_con = C.n_con, n = n_s + n_con;
-  const int j0 = blockIdx.x * kNodes;
+  const int j0 = blockIdx.x * NODES;
   if (j0 >= n_s) return;
   const double beta = C.beta;
   const double eps = 2.220446049250313e-16;
@@ -391,13 +394,13 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
   for (int t = 0; t < C.n_targets; ++t)
     if (C.tkind[t] != 0) Q += C.tK[t] * C.tN[t];
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double* vals = reinterpret_cast<double*>(lds_raw);  // [Q][kNodes]
-  double* mars = vals + (size_t)Q * kNodes;
+  double* vals = reinterpret_cast<double*>(lds_raw);  // [Q][NODES]
+  double* mars = vals + (size_t)Q * NODES;
   // A work unit is kRun consecutive savings nodes of one child row: the
   // queries rise with the node, so each bracket search gallops from the
   // previous node's answer instead of starting over.
   constexpr int kRun = 4;
-  constexpr int kUnitsPerRow = kNodes / kRun;
+  constexpr int kUnitsPerRow = NODES / kRun;
   for (int u = tid; u < Q * kUnitsPerRow; u += kThreads) {
     const int q = u / kUnitsPerRow, jj0 = (u - q * kUnitsPerRow) * kRun;
     if (j0 + jj0 >= n_s) continue;
@@ -418,13 +421,13 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
       read_pair(C.tR[t][j], rw, valid, hint, val, mar);
       mar = mar * C.tG[t][j];
       if (isneginf_(val)) mar = 0.0;
-      vals[q * kNodes + jj] = val;
-      mars[q * kNodes + jj] = mar;
+      vals[q * NODES + jj] = val;
+      mars[q * NODES + jj] = mar;
     }
   }
   __syncthreads();
   const int j = j0 + tid;
-  if (tid >= kNodes || j >= n_s) return;
+  if (tid >= NODES || j >= n_s) return;
   double EV = 0.0, EMU = 0.0;
   int row0 = 0;
   for (int t = 0; t < C.n_targets; ++t) {
@@ -436,9 +439,9 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
     const int K = C.tK[t], N = C.tN[t];
     double vs = 0.0, ms = 0.0;
     for (int k = 0; k < K; ++k) {
-      const int off = (row0 + k * N) * kNodes + tid;
+      const int off = (row0 + k * N) * NODES + tid;
       double sv, sm;
-      aggregate_rows(vals + off, mars + off, kNodes, N, C.tscale[t], sv, sm);
+      aggregate_rows(vals + off, mars + off, NODES, N, C.tscale[t], sv, sm);
       const double w = C.tW[t][k];
       vs = vs + zero_safe(w, sv);
       ms = ms + zero_safe(w, sm);
@@ -943,9 +946,21 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
     return fail(LCM_E_ARG, "lcm_egm_launch: the candidate rows do not fit in this device's LDS");
   if (max_rows < 0 || max_rows > LCM_EGM_MAX_ROWS_PER_COL || max_s < 1 || max_s > LCM_EGM_MAX_CAND)
     return fail(LCM_E_ARG, "lcm_egm_launch: row or node count out of range");
-  const size_t lds_euler = (size_t)max_rows * egm::kNodes * 2 * sizeof(double);
-  hipLaunchKernelGGL(egm::egm_euler_kernel, dim3((max_s + egm::kNodes - 1) / egm::kNodes, n_cols),
-                     dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
+  // Nodes per Euler block: more per block when the launch has workgroups to
+  // spare (the aggregation phase then keeps more of the block's threads busy),
+  // 64 when the launch is small (a single solve: more, shorter blocks).
+  int nodes = egm::kNodesMin;
+  while (nodes < egm::kNodesMax && (size_t)max_rows * (2 * nodes) * 2 * sizeof(double) + 64 <= (size_t)max_lds &&
+         (size_t)n_cols * ((max_s + 2 * nodes - 1) / (2 * nodes)) >= 4096)
+    nodes *= 2;
+  const size_t lds_euler = (size_t)max_rows * nodes * 2 * sizeof(double);
+  const dim3 egrid((max_s + nodes - 1) / nodes, n_cols);
+  if (nodes == 256)
+    hipLaunchKernelGGL(egm::egm_euler_kernel<256>, egrid, dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
+  else if (nodes == 128)
+    hipLaunchKernelGGL(egm::egm_euler_kernel<128>, egrid, dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
+  else
+    hipLaunchKernelGGL(egm::egm_euler_kernel<64>, egrid, dim3(egm::kThreads), lds_euler, ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
   hipLaunchKernelGGL(egm::egm_column_kernel, dim3(n_cols), dim3(egm::kThreads), lds_bytes, ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
