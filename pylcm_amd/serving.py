@@ -371,12 +371,13 @@ class SolveServer:
         self.inflight = still
         return True
 
-    def _split(self, req, t):
+    @classmethod
+    def _split(cls, req, t):
         """The 64-state tasks of 256-state tasks `t`: each tile's four
         quarters that start below its descriptor's n_w."""
-        k = self.COARSE_TILE // self.FINE_TILE
+        k = cls.COARSE_TILE // cls.FINE_TILE
         r = np.repeat(t, k)
-        r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * self.FINE_TILE, len(t))
+        r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * cls.FINE_TILE, len(t))
         return r[r["w0"] < req.n_w[r["desc"] - req.desc_base]]
 
     def _retire(self, block: bool = False):

@@ -131,7 +131,7 @@ def test_split_tasks_are_the_64_state_tiling():
             if len(tc):
                 rec = np.zeros(len(tc), dtype=TASK_DTYPE)
                 rec["desc"], rec["sd"], rec["w0"] = tc[:, 0] + base, tc[:, 1], tc[:, 2]
-                out = SolveServer._split(SolveServer, req, rec)
+                out = SolveServer._split(req, rec)
                 got = {(int(a["desc"]), int(a["sd"]), int(a["w0"])) for a in out}
             assert got == want
 
