@@ -80,3 +80,50 @@ def test_graph_replays_match_eager(native):
         prog.out_host[:] = 0
         prog.run(use_graph=True)
         assert np.array_equal(prog.fetch(), ref)
+
+
+# the kernel's other coordinate modes and variants, each against the oracle
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_log_spaced_wealth_grid(native, dtype):
+    """Log-spaced next-state grid: the coordinate goes through log."""
+    from tests import models as tm
+
+    m = tm.log_grid_model(5)
+    _check(m, ex.retirement_params(5, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0}), dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_irregular_wealth_grid(native, dtype):
+    """Irregular next-state grid: the coordinate is a bracket search."""
+    from tests import models as tm
+
+    m = tm.irregular_grid_model(5)
+    _check(m, ex.retirement_params(5), dtype)
+
+
+def test_state_dependent_tables_f64(native):
+    """Utility and next wealth nonlinear in wealth: tables over the full state
+    grid (the general kernel variant)."""
+    from tests import models as tm
+
+    m = tm.nonaffine_model(5)
+    _check(m, ex.retirement_params(5, log_utility=True), np.float64)
+
+
+def test_taste_shocks_solve_f64(native):
+    """EV1 taste shocks on the discrete action: logsum value, per-cell argmax."""
+    from tests import models as tm
+
+    m = tm.taste_shock_model(5)
+    p = ex.retirement_params(5, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0})
+    p["working_life"]["taste_shocks"] = {"scale": 0.3}
+    _check(m, p, np.float64, taste_regimes=("working_life",))
+
+
+@pytest.mark.parametrize("dist", ["lognormal", "normal", "rouwenhorst", "tauchen", "uniform"])
+def test_processes_model_solve_f64(native, dist):
+    """Markov health transitions and an income process."""
+    from tests import models as tm
+
+    m = tm.processes_model(5, dist)
+    _check(m, tm.processes_params(dist), np.float64)
