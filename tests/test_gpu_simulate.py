@@ -139,6 +139,34 @@ def test_simulate_ev_rows_staged_in_lds_f64(native):
     _compare(m, p, ic, np.float64, seed=10, rtol=1e-9)
 
 
+@pytest.mark.parametrize("kind", ["log", "irregular"])
+def test_simulate_nonlinear_wealth_grids_f64(native, kind):
+    """Log-spaced and irregular next-state grids: the simulation's coordinate
+    goes through log / a bracket search; paths equal the oracle's."""
+    m = tm.log_grid_model(5) if kind == "log" else tm.irregular_grid_model(5)
+    p = (ex.retirement_params(5, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0}) if kind == "log"
+         else ex.retirement_params(5))
+    rng = np.random.default_rng(12)
+    n = 200
+    ic = {"wealth": rng.uniform(1, 400, n), "regime_id": rng.integers(0, 2, n), "age": np.zeros(n)}
+    if kind == "log":
+        ic["income"] = rng.normal(0, 0.2, n)
+    _compare(m, p, ic, np.float64, seed=13, rtol=1e-9)
+
+
+def test_simulate_refuses_state_dependent_utility(native):
+    """The simulation evaluates each action's tables at the agent's own state
+    only where utility does not read the continuous state; a model whose
+    utility does is refused, not approximated."""
+    from pylcm_amd.exceptions import UnsupportedModelError
+
+    m = tm.nonaffine_model(4)
+    p = ex.retirement_params(4, log_utility=True)
+    ic = {"wealth": np.array([10.0, 50.0]), "regime_id": np.zeros(2, dtype=int), "age": np.zeros(2)}
+    with pytest.raises(UnsupportedModelError):
+        m.simulate(params=p, initial_conditions=ic, seed=1, dtype=np.float64)
+
+
 def test_simulate_large_panel_f32(native):
     """100k agents of the stochastic model: runs, first-period value equals the
     solve's V where the agent starts on the grid, income moments follow AR(1)."""
