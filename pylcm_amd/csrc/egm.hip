@@ -185,6 +185,19 @@ __device__ int count_le(const double* __restrict__ xp, int valid, double x, int 
   return L;
 }
 
+// A starting point for count_le on a weakly increasing row: x's position
+// by linear interpolation between the row's ends (count_le returns the same
+// answer from any hint; a close one saves most of its dependent loads).
+__device__ __forceinline__ int guess_index(const double* __restrict__ xp, int valid, double x) {
+  if (valid < 2 || !isfin(x)) return valid >> 1;
+  const double lo = xp[0], hi = xp[valid - 1];
+  if (!(hi > lo)) return valid >> 1;
+  const double f = (x - lo) / (hi - lo);
+  if (!(f > 0.0)) return 0;
+  if (!(f < 1.0)) return valid;
+  return (int)(f * (valid - 1));
+}
+
 // Value (Hermite with the marginal row as node slopes) and marginal (linear)
 // reads of one carry row at x, sharing one bracket; `hint` is updated.
 __device__ __forceinline__ void read_pair(double x, const LcmEgmRow& rw, int valid, int& hint, double& val,
@@ -413,7 +426,7 @@ __global__ void __launch_bounds__(kThreads) egm_euler_kernel(const LcmEgmCol* __
     }
     const LcmEgmRow& rw = C.tRows[t][q - base];
     const int valid = *rw.n_valid;
-    int hint = valid >> 1;
+    int hint = guess_index(rw.grid, valid, C.tR[t][j0 + jj0]);
     for (int jj = jj0; jj < jj0 + kRun; ++jj) {
       const int j = j0 + jj;
       if (j >= n_s) break;
@@ -869,7 +882,6 @@ __global__ void __launch_bounds__(kThreads) egm_publish_kernel(const LcmEgmCol* 
   const double* ov = C.out_value;
   const double* om = C.out_marg;
   const double g0 = og[0];
-  int w_hint = 0;
   for (int w = blockIdx.x * kThreads + threadIdx.x; w < C.n_w; w += gridDim.x * kThreads) {
     const double x = C.W[w];
     double vint;
@@ -878,8 +890,8 @@ __global__ void __launch_bounds__(kThreads) egm_publish_kernel(const LcmEgmCol* 
     } else if (n_valid == 1) {
       vint = ov[0];
     } else {
-      const int u = count_le(og, n_valid, x, w_hint);
-      w_hint = u;
+      // a thread's wealth nodes are a grid stride apart: a fresh guess each time
+      const int u = count_le(og, n_valid, x, guess_index(og, n_valid, x));
       const int up = u < 1 ? 1 : (u > n_valid - 1 ? n_valid - 1 : u);
       vint = between(x, og[up - 1], og[up], ov[up - 1], ov[up], true, om[up - 1], om[up]);
     }
