@@ -143,3 +143,25 @@ def test_struct_layouts_match_the_library():
     assert lib.lcm_abi_sizes_egm(*(ctypes.byref(x) for x in e)) == 0
     assert [x.value for x in e] == [ctypes.sizeof(_native.LcmEgmRow), ctypes.sizeof(_native.LcmEgmCol),
                                     ctypes.sizeof(_native.LcmEgmAgg)]
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from pylcm_amd import _native
+    from pylcm_amd.exceptions import NativeKernelError
+
+    with pytest.raises(NativeKernelError, match="missing"):
+        _native.load_library(tmp_path / "liblcm_amd.so")
+
+
+def test_solve_without_a_device_raises_instead_of_falling_back():
+    """No CPU execution path: where the device cannot be initialised (this
+    suite's hosts have no GPU) a solve raises NativeKernelError."""
+    from pylcm_amd import _native
+    from pylcm_amd import examples as ex
+    from pylcm_amd.exceptions import NativeKernelError
+
+    if _native.load_library().lcm_init(0) == 0:
+        pytest.skip("a GPU is present")
+    m, p = ex.deterministic_workload()
+    with pytest.raises(NativeKernelError):
+        m.solve(params=p)
