@@ -333,6 +333,11 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       if (__builtin_amdgcn_readfirstlane(int(L.pm[mid] <= L.w_wave))) a = mid + 1;
       else b = mid;
     }
+    // Every action here is feasible and its Q is not NaN (the folded loop's
+    // gates), so a quad updates with its maximum alone and remembers only
+    // which quad improved; the position inside that quad is recovered once,
+    // after the loop, from the same Q values recomputed.
+    bool open = false;  // the best quad was taken here and its slot is not known yet
     for (; i + 3 * SPLIT < a; i += 4 * SPLIT) {
       const Act<T> a0 = L.act[i];
       const Act<T> a1 = L.act[i + SPLIT];
@@ -342,7 +347,21 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       const T q1 = q_of(a1);
       const T q2 = q_of(a2);
       const T q3 = q_of(a3);
-      update4(q0, q1, q2, q3, L.valid, L.valid, L.valid, L.valid, base + i);
+      const T m = fmax(fmax(q0, q1), fmax(q2, q3));
+      const bool take = L.valid & !(m <= best);
+      best = take ? m : best;
+      it = take ? base + i : it;
+      open |= take;
+    }
+    if (open) {
+      // the first position of the quad that reaches its maximum
+      const int ql = it - base;
+      const T r0 = q_of(L.act[ql]);
+      const T r1 = q_of(L.act[ql + SPLIT]);
+      const T r2 = q_of(L.act[ql + 2 * SPLIT]);
+      const T r3 = q_of(L.act[ql + 3 * SPLIT]);
+      const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
+      slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
     }
   }
   for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
