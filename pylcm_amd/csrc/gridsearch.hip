@@ -305,23 +305,8 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
   };
   // In the four-action loops the maximiser is kept as (it, slot): the loop
   // position of the last quad that improved and the position k in that quad,
-  // idx = it + k * SPLIT; the slot select takes an inline constant.
+  // idx = it + k * SPLIT.
   int it = idx, slot = 0;
-  auto update4 = [&](T q0, T q1, T q2, T q3, bool f0, bool f1, bool f2, bool f3, int c) {
-    const bool t0 = f0 & !(q0 <= best);
-    best = t0 ? q0 : best;
-    slot = t0 ? 0 : slot;
-    const bool t1 = f1 & !(q1 <= best);
-    best = t1 ? q1 : best;
-    slot = t1 ? 1 : slot;
-    const bool t2 = f2 & !(q2 <= best);
-    best = t2 ? q2 : best;
-    slot = t2 ? 2 : slot;
-    const bool t3 = f3 & !(q3 <= best);
-    best = t3 ? q3 : best;
-    slot = t3 ? 3 : slot;
-    it = (t0 | t1 | t2 | t3) ? c : it;
-  };
   if (__builtin_amdgcn_ballot_w64(L.valid) == 0) return;  // no state of this wave is real
   int i = L.grp;
   if constexpr (LO && !HI) {
@@ -364,6 +349,7 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
     }
   }
+  bool open_b = false;  // as `open` in the prefix loop
   for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
     const Act<T> a0 = L.act[i];
     const Act<T> a1 = L.act[i + SPLIT];
@@ -374,11 +360,31 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     const bool f2 = feas_of(a2, i + 2 * SPLIT);
     const bool f3 = feas_of(a3, i + 3 * SPLIT);
     if (__builtin_amdgcn_ballot_w64(f0 | f1 | f2 | f3) == 0) continue;  // no lane of this wave takes any of the four
-    const T q0 = q_of(a0);
-    const T q1 = q_of(a1);
-    const T q2 = q_of(a2);
-    const T q3 = q_of(a3);
-    update4(q0, q1, q2, q3, f0, f1, f2, f3, base + i);
+    // the same quad update with the infeasible actions as NaN (which the
+    // maximum skips); Q itself is never NaN in the folded loop
+    const T q0 = f0 ? q_of(a0) : T(NAN);
+    const T q1 = f1 ? q_of(a1) : T(NAN);
+    const T q2 = f2 ? q_of(a2) : T(NAN);
+    const T q3 = f3 ? q_of(a3) : T(NAN);
+    const T m = fmax(fmax(q0, q1), fmax(q2, q3));
+    const bool take = (f0 | f1 | f2 | f3) & !(m <= best);
+    best = take ? m : best;
+    it = take ? base + i : it;
+    open_b |= take;
+  }
+  if (open_b) {
+    // the first feasible position of the best quad that reaches its maximum
+    const int ql = it - base;
+    const Act<T> b0 = L.act[ql];
+    const Act<T> b1 = L.act[ql + SPLIT];
+    const Act<T> b2 = L.act[ql + 2 * SPLIT];
+    const Act<T> b3 = L.act[ql + 3 * SPLIT];
+    const T r0 = feas_of(b0, ql) ? q_of(b0) : T(NAN);
+    const T r1 = feas_of(b1, ql + SPLIT) ? q_of(b1) : T(NAN);
+    const T r2 = feas_of(b2, ql + 2 * SPLIT) ? q_of(b2) : T(NAN);
+    const T r3 = feas_of(b3, ql + 3 * SPLIT) ? q_of(b3) : T(NAN);
+    const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
+    slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
   }
   idx = it + slot * SPLIT;
   for (; i < L.cn; i += SPLIT) {
