@@ -232,6 +232,85 @@ def draw_batch_params(n_draws, n_periods=40, seed=0):
 
 
 # ---------------------------------------------------------------------------
+# mortality: stochastic regime transitions
+# ---------------------------------------------------------------------------
+
+
+def next_regime_from_working_mortality(labor_supply, period, survival_probs):
+    """[P(working), P(retired), P(dead)]: survive with `survival_probs[period]`,
+    retire if the agent chose to (`lcm_examples/mortality.py:73-94`)."""
+    sp = survival_probs[period]
+    retire = labor_supply == LaborSupply.retire
+    return np.where(retire, np.array([0.0, sp, 1.0 - sp]), np.array([sp, 0.0, 1.0 - sp]))
+
+
+def next_regime_from_retirement_mortality(period, survival_probs):
+    sp = survival_probs[period]
+    return np.array([0.0, sp, 1.0 - sp])
+
+
+def mortality_model(n_periods: int, *, n_wealth: int = 100, n_consumption: int = 500) -> Model:
+    """Working life, retirement and death with age-dependent survival
+    (`lcm_examples/mortality.py:117-170`): ages 40, 50, ... in 10-year
+    periods, log utility with work disutility, regime transitions given as
+    probability vectors (`MarkovTransition`), so the continuation blends the
+    alive regime with the dead one by the survival probability."""
+    from pylcm_amd.regime import MarkovTransition
+
+    ages = AgeGrid(start=40, stop=40 + (n_periods - 1) * 10, step="10Y")
+    last_age = ages.exact_values[-1]
+    W = LinSpacedGrid(start=1, stop=400, n_points=n_wealth)
+    Cg = LinSpacedGrid(start=1, stop=400, n_points=n_consumption)
+    working = Regime(
+        actions={"labor_supply": DiscreteGrid(LaborSupply), "consumption": Cg},
+        states={"wealth": W},
+        state_transitions={"wealth": next_wealth},
+        constraints={"borrowing_constraint": borrowing_constraint},
+        transition=MarkovTransition(next_regime_from_working_mortality),
+        functions={"utility": log_utility_working, "labor_income": labor_income, "is_working": is_working},
+        active=lambda age, la=last_age: age < la,
+    )
+    retired = Regime(
+        actions={"consumption": Cg},
+        states={"wealth": W},
+        state_transitions={"wealth": next_wealth},
+        constraints={"borrowing_constraint": borrowing_constraint},
+        transition=MarkovTransition(next_regime_from_retirement_mortality),
+        functions={"utility": log_utility_retirement},
+        active=lambda age, la=last_age: age < la,
+    )
+    return Model(regimes={"working_life": working, "retirement": retired, "dead": _dead()},
+                 ages=ages, regime_id_class=RegimeId)
+
+
+def default_survival_probs(n_periods: int) -> np.ndarray:
+    """0.98 down to 0.82 linearly over the first n-2 periods, then 0 (certain
+    death before the last period) (`lcm_examples/mortality.py:173-192`)."""
+    if n_periods <= 2:
+        return np.array([0.0])
+    if n_periods == 3:
+        return np.array([0.98, 0.0])
+    return np.concatenate([np.linspace(0.98, 0.82, n_periods - 2), [0.0]])
+
+
+def mortality_params(n_periods: int, *, discount_factor: float = 0.95, disutility_of_work: float = 0.5,
+                     interest_rate: float = 0.05, wage: float = 10.0, survival_probs=None) -> dict:
+    """Parameters of `mortality_model` (`lcm_examples/mortality.py:195-235`)."""
+    if survival_probs is None:
+        survival_probs = default_survival_probs(n_periods)
+    return {
+        "discount_factor": discount_factor,
+        "interest_rate": interest_rate,
+        "survival_probs": np.asarray(survival_probs, dtype=np.float64),
+        "working_life": {
+            "utility": {"disutility_of_work": disutility_of_work},
+            "labor_income": {"wage": wage},
+        },
+        "retirement": {"next_wealth": {"labor_income": 0.0}},
+    }
+
+
+# ---------------------------------------------------------------------------
 # DC-EGM models
 # ---------------------------------------------------------------------------
 
