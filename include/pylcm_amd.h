@@ -292,6 +292,22 @@ int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, in
                       int32_t* d_counts, int32_t* d_order);
 size_t lcm_sim_lds_bytes(int dtype, int max_g, int max_ev_rows, int max_act);
 
+/*
+ * lcm_sim_period_ex without the caller's scratch: the partition counters and
+ * agent order live in a static device array of the library (16 MiB per
+ * device, held until the process ends), so the call allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Returns LCM_E_ARG when n_regimes * n_agents exceeds
+ * LCM_SIM_NOALLOC_MAX_ORDER. Every call on a device shares that device's
+ * array: periods of one simulation are issued in order from 0 (period 0
+ * clears the counters), and simulations that overlap on one device race.
+ */
+#define LCM_SIM_NOALLOC_MAX_ORDER (1 << 22)
+int lcm_sim_period_noalloc(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                           const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
+                           int n_agents, int period, uint64_t seed, size_t lds_bytes);
+
 /* Raw Philox4x32-10 blocks: out[4*i .. 4*i+3] for counter (i, c1, c2, c3) and
  * key (k0, k1); for known-answer tests of the device generator. */
 int lcm_philox(lcm_ctx* ctx, uint32_t* d_out, int n, uint32_t c1, uint32_t c2, uint32_t c3,

@@ -70,6 +70,7 @@ class LcmGsTask(C.Structure):
 
 SIM_MAX_FEAT = 4
 SIM_MAX_REGIMES = 8
+SIM_NOALLOC_MAX_ORDER = 1 << 22  # regimes x agents of lcm_sim_period_noalloc
 
 
 class LcmEvDesc(C.Structure):
@@ -177,6 +178,8 @@ _SIGS = {
     "lcm_ev_launch": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),
     "lcm_sim_period_ex": ([vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_uint64,
                            C.c_size_t, vp, vp], C.c_int),
+    "lcm_sim_period_noalloc": ([vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_uint64,
+                                C.c_size_t], C.c_int),
     "lcm_sim_lds_bytes": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
     "lcm_egm_launch": ([vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int], C.c_int),
     "lcm_egm_aggregate": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),

@@ -497,7 +497,17 @@ def _initial_conditions(model, ic):
 
 
 class SimProgram:
-    def __init__(self, model, params, prog: GsProgram):
+    """Device simulation of one solved program.
+
+    With `static_scratch=True` every period goes through
+    `lcm_sim_period_noalloc`, whose partition scratch is the library's static
+    device array (for callers that cannot allocate device memory: at most
+    `nat.SIM_NOALLOC_MAX_ORDER` regimes x agents); otherwise through
+    `lcm_sim_period_ex` with scratch in the program's own buffer.
+    """
+
+    def __init__(self, model, params, prog: GsProgram, *, static_scratch: bool = False):
+        self.static_scratch = static_scratch
         self.model = model
         self.params = params
         self.prog = prog
@@ -694,10 +704,14 @@ class SimProgram:
             cur, nxt = A[t % 2], A[(t + 1) % 2]
             if events:
                 events[1][t][0].record(ctx)
-            rc = L.lcm_sim_period_ex(ctx.handle, code, C.c_void_p(b2 + st["sim_desc_off"] + t * R * st["sim_desc_size"]),
-                                     R, C.byref(cur), C.byref(nxt), C.byref(O), n, t, C.c_uint64(seed),
-                                     C.c_size_t(st["lds_max"]), C.c_void_p(base + off["counts"]),
-                                     C.c_void_p(base + off["order"]))
+            descs = C.c_void_p(b2 + st["sim_desc_off"] + t * R * st["sim_desc_size"])
+            if self.static_scratch:
+                rc = L.lcm_sim_period_noalloc(ctx.handle, code, descs, R, C.byref(cur), C.byref(nxt), C.byref(O),
+                                              n, t, C.c_uint64(seed), C.c_size_t(st["lds_max"]))
+            else:
+                rc = L.lcm_sim_period_ex(ctx.handle, code, descs, R, C.byref(cur), C.byref(nxt), C.byref(O), n, t,
+                                         C.c_uint64(seed), C.c_size_t(st["lds_max"]),
+                                         C.c_void_p(base + off["counts"]), C.c_void_p(base + off["order"]))
             nat._check(rc, f"lcm_sim_period (period {t})")
             if events:
                 events[1][t][1].record(ctx)

@@ -3,6 +3,8 @@
 // state transition. Declarations and reference citations: include/pylcm_amd.h.
 
 #include <climits>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "common.h"
@@ -671,10 +673,13 @@ extern "C" size_t lcm_sim_lds_bytes(int dtype, int max_g, int max_ev_rows, int m
   return (ev + size_t(max_g > 0 ? max_g : 1) + size_t(max_act) * kSimChunk) * sz + 64;
 }
 
-extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
-                                 const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
-                                 int n_agents, int period, uint64_t seed, size_t lds_bytes,
-                                 int32_t* d_counts, int32_t* d_order) {
+namespace lcm {
+
+// One period of the simulation with the partition scratch at d_counts (two
+// halves of n_regimes counters) and d_order (n_regimes * n_agents indices).
+static int sim_period(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                      const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out, int n_agents,
+                      int period, uint64_t seed, size_t lds_bytes, int32_t* d_counts, int32_t* d_order) {
   if (!ctx || !d_sim_descs || !cur || !next || !out || !d_counts || !d_order)
     return fail(LCM_E_ARG, "lcm_sim_period: null argument");
   if (n_agents <= 0 || n_regimes <= 0 || n_regimes > LCM_SIM_MAX_REGIMES)
@@ -714,4 +719,65 @@ extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_si
   }
   LCM_HIP(hipGetLastError());
   return LCM_OK;
+}
+
+// Partition scratch of lcm_sim_period_noalloc, one copy per device for the
+// life of the process: kNoallocCountsLen counters (two halves of at most
+// LCM_SIM_MAX_REGIMES, padded so the order array starts 256-byte aligned),
+// then LCM_SIM_NOALLOC_MAX_ORDER agent indices (16 MiB).
+constexpr int kNoallocCountsLen = 64;
+static __device__ __align__(256) int32_t scratch[kNoallocCountsLen + LCM_SIM_NOALLOC_MAX_ORDER];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on ctx's device, looked up with
+// hipGetSymbolAddress on the first call for that device (in host code the
+// array's name is not a device address).
+static int scratch_of(lcm_ctx* ctx, int32_t** out) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices) return fail(LCM_E_ARG, "lcm_sim_period_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = 0;
+    LCM_HIP(hipGetDevice(&prev));
+    if (prev != dev) LCM_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != dev) LCM_HIP(hipSetDevice(prev));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(LCM_E_LAUNCH, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LCM_OK;
+}
+
+}  // namespace lcm
+
+extern "C" int lcm_sim_period_ex(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                                 const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
+                                 int n_agents, int period, uint64_t seed, size_t lds_bytes,
+                                 int32_t* d_counts, int32_t* d_order) {
+  return sim_period(ctx, dtype, d_sim_descs, n_regimes, cur, next, out, n_agents, period, seed, lds_bytes,
+                    d_counts, d_order);
+}
+
+extern "C" int lcm_sim_period_noalloc(lcm_ctx* ctx, int dtype, const LcmSimDesc* d_sim_descs, int n_regimes,
+                                      const LcmAgents* cur, const LcmAgents* next, const LcmSimOut* out,
+                                      int n_agents, int period, uint64_t seed, size_t lds_bytes) {
+  if (!ctx) return fail(LCM_E_ARG, "lcm_sim_period_noalloc: null ctx");
+  if (n_agents <= 0 || n_regimes <= 0 || n_regimes > LCM_SIM_MAX_REGIMES)
+    return fail(LCM_E_ARG, "lcm_sim_period_noalloc: bad sizes");
+  if (int64_t(n_regimes) * int64_t(n_agents) > int64_t(LCM_SIM_NOALLOC_MAX_ORDER))
+    return fail(LCM_E_ARG, "lcm_sim_period_noalloc: n_regimes * n_agents = " +
+                               std::to_string(int64_t(n_regimes) * int64_t(n_agents)) + " exceeds the " +
+                               std::to_string(LCM_SIM_NOALLOC_MAX_ORDER) +
+                               " entries of the static scratch; use lcm_sim_period_ex with a caller buffer");
+  int32_t* base = nullptr;
+  const int rc = scratch_of(ctx, &base);
+  if (rc != LCM_OK) return rc;
+  return sim_period(ctx, dtype, d_sim_descs, n_regimes, cur, next, out, n_agents, period, seed, lds_bytes, base,
+                    base + kNoallocCountsLen);
 }
