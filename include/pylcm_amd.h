@@ -223,6 +223,11 @@ int lcm_memcpy_h2d(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
 int lcm_memcpy_d2h(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
 int lcm_memcpy_h2d_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
 int lcm_memcpy_d2h_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy on ctx's stream: a value function handed to a period
+ * kernel from another buffer is copied into the slot its descriptors read
+ * (the reference passes `next_regime_to_V_arr` as XLA buffers,
+ * src/_lcm/solution/backward_induction.py:391-440). */
+int lcm_memcpy_d2d_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes);
 int lcm_memset_async(lcm_ctx* ctx, void* dst, int value, size_t bytes);
 
 /*
@@ -426,6 +431,12 @@ typedef struct LcmEgmAgg {
  * Columns of one call must not read each other's outputs. */
 int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
                    int max_s);
+/* lcm_egm_launch with the savings nodes per continuation workgroup fixed:
+ * 64, 128 or 256 (0 = chosen per launch as lcm_egm_launch does: more nodes
+ * per workgroup when the launch has thousands of workgroups). Every choice
+ * computes the same values; the choice only moves work between threads. */
+int lcm_egm_launch_ex(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
+                      int max_s, int nodes_per_block);
 int lcm_egm_aggregate(lcm_ctx* ctx, int dtype, const LcmEgmAgg* d_aggs, int n_aggs, int max_w);
 size_t lcm_egm_lds_bytes(int n_cand);
 int lcm_abi_sizes_egm(size_t* row, size_t* col, size_t* agg);

@@ -78,9 +78,10 @@ def _columns(lay, rg):
 class EgmProgram:
     """Backward induction of one (model, params) with DC-EGM regimes."""
 
-    def __init__(self, model, params, dtype, ctx=None, device=True):
+    def __init__(self, model, params, dtype, ctx=None, device=True, nodes_per_block: int = 0):
         if device:
             _abi_check()
+        self.nodes_per_block = int(nodes_per_block)  # 0: chosen per launch by the library
         self.model = model
         self.params = params
         self.dt = np.dtype(dtype)
@@ -305,7 +306,7 @@ class EgmProgram:
                     ic_n=ic.shape[1], R=R_off, G=G_off, n_s=n_s, n_w=n_w, n_con=n_con, n_pad=n_pad,
                     n_cols=n_cols, beta=beta, ratio=ratio, jump=float(solver.envelope.jump_thresh),
                     n_scan=0 if n_scan is None else int(n_scan), w_max=float(np.max(W)),
-                    targets=targets, out=self.col_out[(t, r)], n_rows=n_rows)
+                    targets=targets, out=self.col_out[(t, r)], n_rows=n_rows, regime=r)
         self._col_specs.setdefault(t, []).append(spec)
         self._agg_specs.setdefault(t, []).append((t, r, scale))
         self.max_cand = max(self.max_cand, n_cand)
@@ -425,12 +426,42 @@ class EgmProgram:
         for t in sorted(self._period, reverse=True):
             col_off, n_cols, agg_off, n_aggs = self._period[t]
             lds, max_w, max_rows, max_s = self.lds_by_t[t]
-            nat._check(L.lcm_egm_launch(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
-                                        max_rows, max_s), f"lcm_egm_launch (period {t})")
+            nat._check(L.lcm_egm_launch_ex(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
+                                           max_rows, max_s, self.nodes_per_block), f"lcm_egm_launch (period {t})")
             nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
                        f"lcm_egm_aggregate (period {t})")
         if sync:
             self.ctx.sync()
+
+    def launch_regime(self, t: int, r: str):
+        """The columns and aggregates of one (period, regime) alone: the
+        regime's contiguous range of the period's descriptor arrays, launched
+        with the period's sizes (so its arithmetic is that of the whole
+        period's launch)."""
+        L = nat.lib()
+        code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
+        col_off, _, agg_off, _ = self._period[t]
+        c0 = 0
+        spec = None
+        for sp in self._col_specs[t]:
+            if sp["regime"] == r:
+                spec = sp
+                break
+            c0 += sp["n_cols"]
+        if spec is None:
+            raise KeyError(f"regime {r!r} has no DC-EGM columns at period {t}")
+        a0 = 0
+        for tt, rr, _ in self._agg_specs[t]:
+            if rr == r:
+                break
+            a0 += self.col_out[(tt, rr)]["n_sd"]
+        lds, max_w, max_rows, max_s = self.lds_by_t[t]
+        nat._check(L.lcm_egm_launch_ex(self.ctx.handle, C.c_void_p(self.base + col_off + c0 * COL.itemsize),
+                                       spec["n_cols"], C.c_size_t(lds), max_rows, max_s, self.nodes_per_block),
+                   f"lcm_egm_launch ({r!r}, period {t})")
+        nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(self.base + agg_off + a0 * AGG.itemsize),
+                                       self.col_out[(t, r)]["n_sd"], max_w),
+                   f"lcm_egm_aggregate ({r!r}, period {t})")
 
     def _fetch(self, off, nbytes, dtype):
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
@@ -488,8 +519,9 @@ class EgmBatch:
     its own arena (tables, carry rows, scratch) inside one device buffer, so
     its results are those of its own `EgmProgram`."""
 
-    def __init__(self, model, params_list, dtype, ctx=None):
+    def __init__(self, model, params_list, dtype, ctx=None, nodes_per_block: int = 0):
         _abi_check()
+        self.nodes_per_block = int(nodes_per_block)  # 0: chosen per launch by the library
         self.dt = np.dtype(dtype)
         self.ctx = ctx or nat.Context()
         self.progs = [EgmProgram(model, p, dtype, device=False) for p in params_list]
@@ -544,8 +576,8 @@ class EgmBatch:
         for t in sorted(self._fused, reverse=True):
             col_off, n_cols, agg_off, n_aggs = self._fused[t]
             lds, max_w, max_rows, max_s = self.lds_by_t[t]
-            nat._check(L.lcm_egm_launch(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
-                                        max_rows, max_s), f"lcm_egm_launch (period {t})")
+            nat._check(L.lcm_egm_launch_ex(self.ctx.handle, C.c_void_p(base + col_off), n_cols, C.c_size_t(lds),
+                                           max_rows, max_s, self.nodes_per_block), f"lcm_egm_launch (period {t})")
             nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
                        f"lcm_egm_aggregate (period {t})")
         if sync:

@@ -272,6 +272,9 @@ class GsProgram:
         self.bytes_by_step = [0.0] * S
         dcache = {}
         task_grids = {}
+        self.desc_of = {}     # (problem, period, regime) -> descriptor index
+        self.desc_tasks = {}  # descriptor index -> its (sd, w0) task rows
+        self._regime_tasks = {}
         for p, (model, tabs, rg) in enumerate(zip(self.models, self.problems, self._rg)):
             order = sorted(tabs.items(), key=lambda kv: (-kv[0][0], model.regime_code(kv[0][1])))
             for (t, r), pt in order:
@@ -280,6 +283,7 @@ class GsProgram:
                 d = self._describe_cached(model, p, t, r, pt, rg, ar, zero_off, dcache)
                 di = len(descs)
                 descs.append(d)
+                self.desc_of[(p, t, r)] = di
                 wdep[s] = max(wdep[s], d["variant"])
                 kmax[s] = max(kmax[s], d["n_constr"])
                 if d["g_n"] > 0:
@@ -299,6 +303,7 @@ class GsProgram:
                                             np.arange(0, lay.n_w, self.tile, dtype=np.int32), indexing="ij")
                     grid = task_grids[key] = np.stack([sd_i.ravel(), w_i.ravel()], axis=1)
                 tasks[s].append((di, grid))
+                self.desc_tasks[di] = grid
         self.n_desc = len(descs)
         self.desc_off = ar.reserve(self.n_desc * DESC_DTYPE.itemsize)
         self.task_off, self.n_tasks = [], []
@@ -532,6 +537,31 @@ class GsProgram:
             C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]),
         )
         nat._check(st, f"lcm_gs_launch (step {s})")
+
+    def launch_regime(self, p: int, t: int, r: str):
+        """One (problem, period, regime) launch: that descriptor's tasks alone,
+        with its step's tile, LDS size and kernel variant, so its arithmetic
+        is that of the fused launch of the whole step."""
+        di = self.desc_of[(p, t, r)]
+        s = self.step_of(p, t)
+        if di not in self._regime_tasks:
+            grid = self.desc_tasks[di]
+            n = len(grid)
+            npad = (n + 7) // 8 * 8
+            arr = np.zeros(npad, dtype=TASK_DTYPE)
+            arr["desc"][:n] = di
+            arr["sd"][:n] = grid[:, 0]
+            arr["w0"][:n] = grid[:, 1]
+            arr["desc"][n:] = -1
+            buf = nat.DeviceBuffer(arr.nbytes)
+            nat.h2d(self.ctx, buf.ptr, arr)
+            self._regime_tasks[di] = (buf, npad)
+        buf, npad = self._regime_tasks[di]
+        code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
+        st = nat.lib().lcm_gs_launch_ex(
+            self.ctx.handle, code, C.c_void_p(self.base + self.desc_off), C.c_void_p(buf.ptr), npad,
+            self.tile, C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]))
+        nat._check(st, f"lcm_gs_launch ({r!r}, period {t})")
 
     def _record(self):
         for s in range(self.n_steps):

@@ -158,6 +158,7 @@ _SIGS = {
     "lcm_memcpy_d2h": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_memcpy_h2d_async": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_memcpy_d2h_async": ([vp, vp, vp, C.c_size_t], C.c_int),
+    "lcm_memcpy_d2d_async": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_memset_async": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
     "lcm_gs_launch": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_size_t], C.c_int),
     "lcm_gs_launch_ex": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_size_t, C.c_int], C.c_int),
@@ -182,6 +183,7 @@ _SIGS = {
                                 C.c_size_t], C.c_int),
     "lcm_sim_lds_bytes": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
     "lcm_egm_launch": ([vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int], C.c_int),
+    "lcm_egm_launch_ex": ([vp, vp, C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int], C.c_int),
     "lcm_egm_aggregate": ([vp, C.c_int, vp, C.c_int, C.c_int], C.c_int),
     "lcm_egm_lds_bytes": ([C.c_int], C.c_size_t),
     "lcm_abi_sizes_egm": ([C.POINTER(C.c_size_t)] * 3, C.c_int),
@@ -328,6 +330,48 @@ def h2d(ctx: Context, dst: int, arr: np.ndarray):
 def d2h(ctx: Context, src: int, out: np.ndarray):
     assert out.flags.c_contiguous
     call("lcm_memcpy_d2h", ctx.handle, vp(out.ctypes.data), vp(src), C.c_size_t(out.nbytes))
+
+
+def d2d(ctx: Context, dst: int, src: int, nbytes: int):
+    """Device-to-device copy on ctx's stream (asynchronous)."""
+    call("lcm_memcpy_d2d_async", ctx.handle, vp(dst), vp(src), C.c_size_t(int(nbytes)))
+
+
+class DeviceArray:
+    """A device-resident array: `ptr` points into memory that `owner` keeps
+    alive. `np.asarray(a)` (or `a.to_numpy()`) copies it to the host after
+    the work queued on `ctx` has finished."""
+
+    def __init__(self, ctx: Context, ptr: int, shape, dtype, owner=None):
+        self.ctx = ctx
+        self.ptr = int(ptr)
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.owner = owner
+
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.shape, dtype=np.int64)) if self.shape else 1
+
+    @property
+    def nbytes(self) -> int:
+        return self.size * self.dtype.itemsize
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.size, dtype=self.dtype)
+        d2h(self.ctx, self.ptr, out)
+        return out.reshape(self.shape)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.to_numpy()
+        return a if dtype is None else a.astype(dtype)
+
+    def __repr__(self):
+        return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
 
 
 class Event:

@@ -944,9 +944,11 @@ extern "C" size_t lcm_egm_lds_bytes(int n_cand) {
   return (size_t)P * (4 * sizeof(double) + sizeof(int));
 }
 
-extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
-                              int max_s) {
+extern "C" int lcm_egm_launch_ex(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes,
+                                 int max_rows, int max_s, int nodes_per_block) {
   if (!ctx || (!d_cols && n_cols)) return fail(LCM_E_ARG, "lcm_egm_launch: null argument");
+  if (nodes_per_block != 0 && nodes_per_block != 64 && nodes_per_block != 128 && nodes_per_block != 256)
+    return fail(LCM_E_ARG, "lcm_egm_launch: nodes_per_block must be 0 (chosen per launch), 64, 128 or 256");
   if (n_cols <= 0) return LCM_OK;
   if (n_cols > 65535) return fail(LCM_E_ARG, "lcm_egm_launch: more than 65535 columns in one launch");
   if (lds_bytes > lcm_egm_lds_bytes(LCM_EGM_MAX_CAND))
@@ -962,9 +964,15 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
   // spare (the aggregation phase then keeps more of the block's threads busy),
   // 64 when the launch is small (a single solve: more, shorter blocks).
   int nodes = egm::kNodesMin;
-  while (nodes < egm::kNodesMax && (size_t)max_rows * (2 * nodes) * 2 * sizeof(double) + 64 <= (size_t)max_lds &&
-         (size_t)n_cols * ((max_s + 2 * nodes - 1) / (2 * nodes)) >= 4096)
-    nodes *= 2;
+  if (nodes_per_block) {
+    nodes = nodes_per_block;
+    if ((size_t)max_rows * nodes * 2 * sizeof(double) + 64 > (size_t)max_lds)
+      return fail(LCM_E_ARG, "lcm_egm_launch: the continuation rows of that many nodes do not fit in LDS");
+  } else {
+    while (nodes < egm::kNodesMax && (size_t)max_rows * (2 * nodes) * 2 * sizeof(double) + 64 <= (size_t)max_lds &&
+           (size_t)n_cols * ((max_s + 2 * nodes - 1) / (2 * nodes)) >= 4096)
+      nodes *= 2;
+  }
   const size_t lds_euler = (size_t)max_rows * nodes * 2 * sizeof(double);
   const dim3 egrid((max_s + nodes - 1) / nodes, n_cols);
   if (nodes == 256)
@@ -983,6 +991,11 @@ extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols,
                      ctx->stream, d_cols);
   LCM_HIP(hipGetLastError());
   return LCM_OK;
+}
+
+extern "C" int lcm_egm_launch(lcm_ctx* ctx, const LcmEgmCol* d_cols, int n_cols, size_t lds_bytes, int max_rows,
+                              int max_s) {
+  return lcm_egm_launch_ex(ctx, d_cols, n_cols, lds_bytes, max_rows, max_s, 0);
 }
 
 extern "C" int lcm_egm_aggregate(lcm_ctx* ctx, int dtype, const LcmEgmAgg* d_aggs, int n_aggs, int max_w) {

@@ -145,6 +145,13 @@ int lcm_memcpy_d2h_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes)
   return LCM_OK;
 }
 
+int lcm_memcpy_d2d_async(lcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(LCM_E_ARG, "lcm_memcpy_d2d_async: null argument");
+  if (bytes == 0 || dst == src) return LCM_OK;
+  LCM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return LCM_OK;
+}
+
 int lcm_memset_async(lcm_ctx* ctx, void* dst, int value, size_t bytes) {
   if (!ctx || (!dst && bytes)) return fail(LCM_E_ARG, "lcm_memset_async: null argument");
   if (bytes == 0) return LCM_OK;

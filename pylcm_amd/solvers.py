@@ -15,12 +15,30 @@ from pylcm_amd.exceptions import RegimeInitializationError
 
 
 class Solver:
-    """Base class of solver configurations."""
+    """Base class of solver configurations (`src/_lcm/solution/contract.py:549-619`).
+
+    `build_period_kernels(context=...)` returns the regime's `SolutionKernels`
+    (`pylcm_amd.contract`); `requires_continuation` says whether the solver
+    reads its targets' continuation payloads.
+    """
+
+    def build_period_kernels(self, *, context):
+        raise NotImplementedError(f"{type(self).__name__} builds no period kernels")
+
+    @property
+    def requires_continuation(self) -> bool:
+        return False
 
 
 @dataclass(frozen=True, kw_only=True)
 class GridSearch(Solver):
     """Brute-force max over the full action grid (the reference's default)."""
+
+    def build_period_kernels(self, *, context):
+        """One HIP grid-search period kernel per active period (`grid_search.py:99-132`)."""
+        from pylcm_amd.contract import grid_search_kernels
+
+        return grid_search_kernels(context)
 
 
 @dataclass(frozen=True, kw_only=True)
@@ -55,6 +73,16 @@ class DCEGM(Solver):
     refined_grid_factor: float = 2.0
     n_constrained_points: int = 20
     stochastic_node_batch_size: int = 0
+
+    def build_period_kernels(self, *, context):
+        """One HIP DC-EGM period kernel per active period (`dcegm.py:359-419`)."""
+        from pylcm_amd.contract import dcegm_kernels
+
+        return dcegm_kernels(context)
+
+    @property
+    def requires_continuation(self) -> bool:
+        return True
 
     def __post_init__(self):
         if self.refined_grid_factor < 1.0:

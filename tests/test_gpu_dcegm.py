@@ -112,17 +112,20 @@ def test_dcegm_model_solve_api(native):
             assert rel_err(V[t][r], Vo[t][r]) <= 1e-10
 
 
+@pytest.mark.parametrize("nodes", [0, 64, 128, 256])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_dcegm_batch_matches_single_solves(native, dtype):
+def test_dcegm_batch_matches_single_solves(native, dtype, nodes):
     """Parameter sets solved in one fused launch per period give, bit for bit,
     the V, carry rows and kept counts of each set solved alone; the first set
-    also matches the FP64 oracle."""
+    also matches the FP64 oracle. `nodes` fixes the savings nodes per
+    continuation workgroup of the batch (0: chosen per launch), so each
+    kernel variant is compared with the single solve's."""
     from pylcm_amd._egm_program import EgmBatch, EgmProgram
 
     m = ex.dcegm_retirement_model(5, n_savings=200, savings_max=100.0, n_wealth=90, wealth_range=(0.05, 100.0))
     plist = [ex.dcegm_params(5, discount_factor=b, crra=c, sigma=s)
              for b, c, s in [(0.96, 2.0, 0.15), (0.93, 1.5, 0.1), (0.98, 3.0, 0.2)]]
-    batch = EgmBatch(m, plist, dtype)
+    batch = EgmBatch(m, plist, dtype, nodes_per_block=nodes)
     batch.run()
     for i, p in enumerate(plist):
         single = EgmProgram(m, p, dtype)
