@@ -81,6 +81,11 @@ typedef struct LcmGsDesc {
   /* with use_interval: per (sd, d, chunk of LCM_ACT_CHUNK actions) the pair
    * (min Flo, max Fhi), so a block skips chunks no state of it can take */
   const void* Fchunk;
+  /* with a linear next grid: A / step and (B - start) / step, each rounded
+   * once to the model dtype (strides of A and B); the folded loop's
+   * coordinate of the next state is Af * w + Bf */
+  const void* Af;
+  const void* Bf;
   int64_t U_ssd, U_sw, A_ssd, A_sw, B_ssd, B_sw;
   int64_t Fa_ssd[LCM_MAX_CONSTR], Fa_sw[LCM_MAX_CONSTR];
   int64_t Fb_ssd[LCM_MAX_CONSTR], Fb_sw[LCM_MAX_CONSTR];

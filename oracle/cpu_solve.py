@@ -34,7 +34,7 @@ def _dp(a):
 
 
 def solve_c(model, params, *, grid_dtype=np.float64, threads=0, next_V=None, cmp_policy=None,
-            tie_ulps=8):
+            tie_ulps=8, reference_periods=()):
     """FP64 backward induction with the C Bellman step. Returns (V, POL).
 
     With `next_V` (period -> regime -> V) each period reads that V for period
@@ -42,10 +42,18 @@ def solve_c(model, params, *, grid_dtype=np.float64, threads=0, next_V=None, cmp
     period at a time from the same inputs. With `cmp_policy` the return is
     (V, POL, report): `report` counts the policy entries that differ from
     `cmp_policy` as near-ties (this step's FP64 Q at the two actions within
-    `tie_ulps` units in the last place of `grid_dtype`) or genuine mismatches.
+    `tie_ulps` units in the last place of `grid_dtype`) or genuine mismatches,
+    ties split at one ulp and those above graded against the argmax of Q in
+    the reference's own arithmetic at `grid_dtype` (`reference_form_q`). For
+    the periods in `reference_periods` it also counts, over every state, how
+    often that reference-arithmetic argmax and `cmp_policy` each depart from
+    the FP64 argmax by more than one ulp.
     """
     lib = _load()
-    report = {"mismatches": 0, "near_ties": 0, "max_tie_gap_ulps": 0.0}
+    report = {"mismatches": 0, "near_ties": 0, "max_tie_gap_ulps": 0.0, "near_ties_le_1ulp": 0,
+              "near_ties_gt_1ulp": 0, "gt_1ulp_same_as_reference_f32": 0, "gt_1ulp_reference_f32_also_flips": 0,
+              "gt_1ulp_unexplained": 0, "sampled_states": 0, "reference_f32_gt_1ulp_sampled": 0,
+              "compared_gt_1ulp_sampled": 0, "reference_f32_max_gap_ulps_sampled": 0.0}
     eps = float(np.finfo(np.dtype(grid_dtype)).eps)
     gdt = np.dtype(grid_dtype)
     fixed = model.fixed_params
@@ -123,6 +131,44 @@ def solve_c(model, params, *, grid_dtype=np.float64, threads=0, next_V=None, cmp
                 report["near_ties"] += int(np.sum(tie))
                 if np.any(tie):
                     report["max_tie_gap_ulps"] = max(report["max_tie_gap_ulps"], float(np.max(gap[tie])))
+                gt1 = tie & (gap > 1.0)
+                report["near_ties_le_1ulp"] += int(np.sum(tie & ~gt1))
+                report["near_ties_gt_1ulp"] += int(np.sum(gt1))
+                if np.any(gt1) and taste == 0:
+                    # ties beyond one ulp: the argmax of Q in the reference's
+                    # own arithmetic at the compute precision, from the same V
+                    states = bad[gt1]
+                    Qr, Fr = og.reference_form_q(tab, reg, states, compute_dtype=gdt)
+                    a_ref = og.first_argmax(Qr, Fr)
+                    same = a_ref == pcmp[states]
+                    flips = a_ref != pol[states]
+                    report["gt_1ulp_same_as_reference_f32"] += int(np.sum(same))
+                    report["gt_1ulp_reference_f32_also_flips"] += int(np.sum(flips))
+                    report["gt_1ulp_unexplained"] += int(np.sum(~(same | flips)))
+                elif np.any(gt1):
+                    report["gt_1ulp_unexplained"] += int(np.sum(gt1))
+                if t in reference_periods and taste == 0 and not tab["terminal"]:
+                    n_st = nsd * nw
+                    states = np.arange(n_st)
+                    Q64, F64 = og.reference_form_q(tab, reg, states, compute_dtype=np.float64)
+                    Qr, _ = og.reference_form_q(tab, reg, states, compute_dtype=gdt)
+                    a_ref = og.first_argmax(Qr, F64)
+                    Qf = np.where(F64, Q64, -np.inf).reshape(n_st, -1)
+                    pol64 = og.first_argmax(Q64, F64)
+
+                    def n_gt1(a):
+                        diff = np.nonzero((a != pol64) & (a >= 0))[0]
+                        qa, qb = Qf[diff, a[diff]], Qf[diff, pol64[diff]]
+                        with np.errstate(invalid="ignore"):
+                            g = np.abs(qb - qa) / (eps * np.maximum(np.maximum(np.abs(qa), np.abs(qb)), 1.0))
+                        return int(np.sum(g > 1.0)), (float(np.max(g)) if g.size else 0.0)
+
+                    nr, gr = n_gt1(a_ref)
+                    nc_, _ = n_gt1(pcmp.astype(np.int64))
+                    report["sampled_states"] += n_st
+                    report["reference_f32_gt_1ulp_sampled"] += nr
+                    report["compared_gt_1ulp_sampled"] += nc_
+                    report["reference_f32_max_gap_ulps_sampled"] = max(report["reference_f32_max_gap_ulps_sampled"], gr)
             V[t][n] = Vout.reshape(reg.V_shape)
             POL[t][n] = pol.reshape(reg.V_shape + ((nd,) if taste > 0 else ()))
     if cmp_policy is not None:

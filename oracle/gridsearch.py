@@ -583,3 +583,114 @@ def _max_over_actions(Q, F, reg, taste):
         V = fm + taste * np.log(s)
     V = np.where(np.any(np.isnan(qc), axis=1), np.nan, V)
     return V, pol
+
+
+# ----------------------------------------------------------------------------
+# the reference's arithmetic in a lower precision (for grading argmax ties)
+# ----------------------------------------------------------------------------
+
+
+def reference_form_q(tab, reg, states, compute_dtype=np.float32):
+    """Q and F of one (regime, period) at flat states `states` (index
+    `sd * n_w + w`), every arithmetic step rounded to `compute_dtype`, in the
+    reference's order of operations (`Q_and_F.py:466-681,1145-1209`): per
+    target and node the coordinate of the next state, the interpolation
+    `(1 - t) * V[lo] + t * V[lo + 1]` with zero-safe corners
+    (`ndimage.py:73-109`), the node-weighted sum divided by the weight sum,
+    the probability-weighted blend over targets divided by the mass, and
+    `Q = U + beta * CE`.
+
+    `tab` is `tabulate(...)` of the same (regime, period); the user's
+    functions enter as their float64 values rounded to `compute_dtype`. This
+    is how the reference's float32 solve evaluates Q, up to its compiler's
+    choice of fused multiply-adds; argmax ties whose float64 gap exceeds one
+    float32 ulp are graded against it."""
+    cdt = np.dtype(compute_dtype)
+    c = cdt.type
+    shape = tab["shape"]
+    n_sdax = len(reg.sd)
+    nsd = int(np.prod(shape[:n_sdax])) if n_sdax else 1
+    nw = shape[n_sdax]
+    nd = int(np.prod(shape[n_sdax + 1:-1])) if len(shape) > n_sdax + 2 else 1
+    nc = shape[-1]
+    states = np.asarray(states, dtype=np.int64)
+    sd_i, w_i = states // nw, states % nw
+
+    def at_states(a):
+        return np.broadcast_to(a, shape).reshape(nsd, nw, nd, nc)[sd_i, w_i]
+
+    def per_sd_da(p):
+        return np.asarray(p, dtype=np.float64).reshape(nsd, nd)[sd_i][:, :, None].astype(cdt)
+
+    def zs(w, v):
+        return w * np.where((w == c(0)) & ~np.isfinite(v), c(0), v)
+
+    U = at_states(tab["U"]).astype(cdt)
+    F = at_states(tab["F"])
+    if tab["terminal"]:
+        return U, F
+    CE = np.zeros(U.shape, dtype=cdt)
+    mass = np.zeros(U.shape, dtype=cdt)
+    neg = np.zeros(U.shape, dtype=bool)
+    for _, p, v in tab["scalars"]:
+        pg = per_sd_da(p)
+        mass = mass + pg
+        neg |= np.signbit(pg) & (pg != c(0))
+        CE = CE + zs(pg, c(v))
+    if tab["targets"]:
+        grid, pts = tab["grid"]
+        pts = pts.astype(cdt)
+        x = at_states(tab["wn"]).astype(cdt)
+        n = len(pts)
+        kind = type(grid).__name__
+        if n == 1:
+            coord = np.zeros_like(x)
+        elif kind == "LinSpacedGrid":
+            step = (pts[-1] - pts[0]) / c(n - 1)
+            coord = (x - pts[0]) / step
+        elif kind == "LogSpacedGrid":
+            sl, el = np.log(pts[0]), np.log(pts[-1])
+            step = (el - sl) / c(n - 1)
+            cl = (np.log(x) - sl) / step
+            rlo = np.floor(cl)
+            lower, upper = np.exp(sl + step * rlo), np.exp(sl + step * (rlo + c(1)))
+            coord = rlo + (x - lower) / (upper - lower)
+        else:
+            hi = np.clip(np.searchsorted(pts, x, side="right"), 1, n - 1)
+            lo_ = hi - 1
+            coord = lo_.astype(cdt) + (x - pts[lo_]) / (pts[hi] - pts[lo_])
+        with np.errstate(invalid="ignore"):
+            lo = np.clip(np.floor(coord), c(0), c(max(n - 2, 0)))
+        lo_i = np.where(np.isfinite(lo), lo, 0).astype(np.int64)
+        t = coord - lo
+    for _, p, W, Vn in tab["targets"]:
+        pg = per_sd_da(p)
+        mass = mass + pg
+        neg |= np.signbit(pg) & (pg != c(0))
+        Wk = np.asarray(W, dtype=np.float64).reshape(nsd, nd, -1)[sd_i].astype(cdt)
+        num = np.zeros(U.shape, dtype=cdt)
+        wsum = np.zeros(U.shape, dtype=cdt)
+        for k in range(Wk.shape[-1]):
+            row = np.asarray(Vn[k]).astype(cdt)
+            if len(row) == 1:
+                v = np.broadcast_to(row[0], U.shape)
+            else:
+                v = zs(c(1) - t, row[lo_i]) + zs(t, row[lo_i + 1])
+            wk = Wk[:, :, k][:, :, None]
+            num = num + zs(wk, v)
+            wsum = wsum + wk
+        E = num / np.where(wsum > c(0), wsum, c(1))
+        CE = CE + zs(pg, E)
+    ok = (np.abs(mass.astype(np.float64) - 1.0) <= 1e-3) & ~neg
+    CE = CE / np.where(ok, mass, c(np.nan))
+    return U + c(tab["beta"]) * CE, F
+
+
+def first_argmax(Q, F):
+    """First maximiser over the flattened action axes of each state row
+    (`argmax.py:6-69`); -1 where no action is feasible."""
+    k = Q.shape[0]
+    Qa = np.where(F, Q, -np.inf).reshape(k, -1)
+    m = np.max(Qa, axis=1)
+    pol = np.argmax(Qa == m[:, None], axis=1)
+    return np.where(np.isfinite(m), pol, -1)

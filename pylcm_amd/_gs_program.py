@@ -145,6 +145,21 @@ def chunk_bounds(lo, hi, chunk):
     return out
 
 
+def folded_next_state(A, B, g_start, g_stop, g_n, dt):
+    """(A / step, (B - start) / step) of a linear next grid, each rounded once
+    to `dt` from the `dt`-rounded A, B, start and step (step computed in `dt`
+    as the kernel computes it: (stop - start) / (n - 1)). The folded loop's
+    coordinate of the next state is then one fma, Af * w + Bf."""
+    dt = np.dtype(dt)
+    c = dt.type
+    start, stop = c(g_start), c(g_stop)
+    step = c(stop - start) / c(g_n - 1) if g_n > 1 else c(1)
+    A64 = np.asarray(A, dtype=dt).astype(np.float64)
+    B64 = np.asarray(B, dtype=dt).astype(np.float64)
+    return ((A64 / np.float64(step)).astype(dt),
+            ((B64 - np.float64(start)) / np.float64(step)).astype(dt))
+
+
 def default_tile(n_states_total: int) -> int:
     """States per workgroup: small workloads split actions over more waves."""
     if n_states_total <= 64 * 1024:
@@ -420,7 +435,7 @@ class GsProgram:
         d["beta"] = pt.beta
         d["Vn"], d["M"], d["nj"], d["Vs"], d["ms"] = [], [], [], [], []
         d["g_kind"], d["g_n"], d["g_start"], d["g_stop"], d["g_pts"] = 0, 0, 0.0, 0.0, zero_off
-        d["A"] = d["B"] = zero_off
+        d["A"] = d["B"] = d["Af"] = d["Bf"] = zero_off
         d["A_ssd"] = d["A_sw"] = d["B_ssd"] = d["B_sw"] = 0
         if not pt.terminal:
             if len(pt.targets) > nat.MAX_TARGETS or len(pt.scalars) > nat.MAX_SCALAR:
@@ -449,6 +464,10 @@ class GsProgram:
                 d["g_pts"] = ar.add(gpts, dtype=dt)
                 d["A"] = ar.add(pt.A, dtype=dt)
                 d["B"] = ar.add(pt.B, dtype=dt)
+                if d["g_kind"] == 0:
+                    Af, Bf = folded_next_state(pt.A, pt.B, d["g_start"], d["g_stop"], d["g_n"], dt)
+                    d["Af"] = ar.add(Af, dtype=dt)
+                    d["Bf"] = ar.add(Bf, dtype=dt)
                 d["A_ssd"], d["A_sw"] = _strides(pt.A)
                 d["B_ssd"], d["B_sw"] = _strides(pt.B)
                 wdep |= d["A_sw"] != 0 or d["B_sw"] != 0
@@ -481,7 +500,7 @@ class GsProgram:
             D[f][m] += np.uint64(base)
         return D
 
-    _PTR_FIELDS = ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi", "Fchunk")
+    _PTR_FIELDS = ("V_out", "pol_out", "w_pts", "U", "A", "B", "g_pts", "Flo", "Fhi", "Fchunk", "Af", "Bf")
     _VAL_FIELDS = ("U_ssd", "U_sw", "A_ssd", "A_sw", "B_ssd", "B_sw", "n_sd", "n_w", "n_d", "n_c",
                    "n_constr", "strict_mask", "terminal", "g_kind", "g_n", "has_taste",
                    "beta", "taste_scale", "g_start", "g_stop", "Fi_ssd", "Fc_ssd", "use_interval")

@@ -46,6 +46,8 @@ class LcmGsDesc(C.Structure):
         ("Flo", vp),
         ("Fhi", vp),
         ("Fchunk", vp),
+        ("Af", vp),
+        ("Bf", vp),
         ("U_ssd", C.c_int64), ("U_sw", C.c_int64),
         ("A_ssd", C.c_int64), ("A_sw", C.c_int64),
         ("B_ssd", C.c_int64), ("B_sw", C.c_int64),

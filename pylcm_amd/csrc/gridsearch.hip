@@ -187,6 +187,16 @@ struct alignas(sizeof(T) * 2) Pair {
   T v, dv;
 };
 
+// a * b + c with one rounding, written out so that every use of an
+// expression rounds the same way whatever the compiler's contraction choices.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The accumulator of the EV contraction: FP64 whatever T, so the row is the
+// contraction rounded once to T.
+template <typename T>
+using EvAcc = double;
+
 // Everything the action loop reads (block-uniform except w / valid / wr).
 template <typename T>
 struct LoopArgs {
@@ -291,6 +301,11 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     return f;
   };
   const T beta = L.beta;
+  // Written as plain arithmetic: explicit fmas here (__builtin_fmaf, or
+  // `fp contract(on)`) schedule 4 % slower than the default contraction. The
+  // recovery of the best quad's slot after each loop recomputes Q with this
+  // function and takes V from those same recomputed values, so V and the
+  // policy index agree whatever the contraction.
   auto q_of = [&](const Act<T>& a) -> T {
     if constexpr (!INTERP) return a.U;  // staged as U + beta * cs
     const T coord = a.A * w + a.B;
@@ -347,6 +362,7 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       const T r3 = q_of(L.act[ql + 3 * SPLIT]);
       const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
       slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
+      best = mr;  // V from the values the slot was chosen from
     }
   }
   bool open_b = false;  // as `open` in the prefix loop
@@ -385,6 +401,7 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
     const T r3 = feas_of(b3, ql + 3 * SPLIT) ? q_of(b3) : T(NAN);
     const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
     slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
+    best = mr;  // V from the values the slot was chosen from
   }
   idx = it + slot * SPLIT;
   for (; i < L.cn; i += SPLIT) {
@@ -606,21 +623,23 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         const T v = static_cast<const T*>(D->Vs[s])[0];
         cs += zero_safe(m, v);
       }
-      // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k]   (accumulated in T)
+      // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k], accumulated
+      // in FP64 and rounded once to T (a few FP64 fmas per node and block)
       int local_nonfinite = 0;
       for (int k = tid; k < g_n; k += kBlock) {
-        T acc = T(0);
+        EvAcc<T> acc = 0.0;
         for (int r = 0; r < D->n_targets; ++r) {
           const int nj = D->n_sd_next[r];
           const T* __restrict__ Mr = static_cast<const T*>(D->M[r]) + (int64_t(sd) * n_d + d) * nj;
           const T* __restrict__ Vr = static_cast<const T*>(D->Vn[r]);
           for (int j = 0; j < nj; ++j) {
             const T m = Mr[j];
-            if (m != T(0)) acc += m * Vr[int64_t(j) * g_n + k];
+            if (m != T(0)) acc = fma_t(double(m), double(Vr[int64_t(j) * g_n + k]), acc);
           }
         }
-        ev[k * ES] = acc;
-        local_nonfinite |= !is_finite(acc);
+        const T ev_k = T(acc);
+        ev[k * ES] = ev_k;
+        local_nonfinite |= !is_finite(ev_k);
       }
       finite_row = __syncthreads_or(local_nonfinite) == 0;
       if (FAST && mode == kModeInterp && finite_row && g_n >= 2) {
@@ -640,10 +659,12 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
     L.Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
     L.Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
     L.Bg = interp ? static_cast<const T*>(D->B) + int64_t(sd) * D->B_ssd + wofs_b + int64_t(d) * n_c : nullptr;
+    // the host-folded tables of the folded loop (no continuous-state axis there)
+    const T* Afg = interp ? static_cast<const T*>(D->Af) + int64_t(sd) * D->A_ssd + int64_t(d) * n_c : nullptr;
+    const T* Bfg = interp ? static_cast<const T*>(D->Bf) + int64_t(sd) * D->B_ssd + int64_t(d) * n_c : nullptr;
     // the common cases run the folded loop: interpolation with the grid
     // folded into A, B and the EV row scaled, or a scalar continuation
     const bool folded = FAST && ((mode == kModeInterp && finite_row && row_small) || mode == kModeScalar);
-    const T bcs = L.beta * cs;
 
     for (int c0 = 0; c0 < n_c; c0 += CH) {
       const int cn = min(CH, n_c - c0);
@@ -665,21 +686,16 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
           for (int i = tid; i < cn; i += kBlock) {
             Act<T> a;
             if (interp) {
-              const T A = L.Ag[c0 + i];
-              const T B = L.Bg[c0 + i] - g.start;
-              if constexpr (std::is_same<T, double>::value) {
-                a.A = A / g.step;
-                a.B = B / g.step;
-              } else {
-                a.A = A * g.inv_step;
-                a.B = B * g.inv_step;
-              }
-              a.U = L.Ug[c0 + i] + bcs;
+              // A' = A / step and B' = (B - start) / step come folded by the
+              // host (each rounded once); U' = U + beta * cs is one fma
+              a.A = Afg[c0 + i];
+              a.B = Bfg[c0 + i];
+              a.U = fma_t(L.beta, cs, L.Ug[c0 + i]);
               // |coord| over the block's states (w_pts increase) and U' not NaN
               unsafe |= !(fabs(a.A) * w_abs + fabs(a.B) <= T(4194304)) | (a.U != a.U);
             } else {
               a.A = a.B = T(0);
-              a.U = L.Ug[c0 + i] + bcs;  // Q of a scalar continuation
+              a.U = fma_t(L.beta, cs, L.Ug[c0 + i]);  // Q of a scalar continuation
               unsafe |= a.U != a.U;
             }
             a.lo = static_cast<const T*>(D->Flo)[fi + i];

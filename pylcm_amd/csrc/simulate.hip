@@ -94,18 +94,20 @@ ev_kernel(const LcmEvDesc* __restrict__ descs, int max_rows) {
   T* out = static_cast<T*>(E.EV) + int64_t(row) * g_n;
   int nonfinite = 0;
   for (int k = threadIdx.x; k < g_n; k += blockDim.x) {
-    T acc = T(0);
+    // accumulated in FP64 and rounded once to T, as the solve's EV rows
+    double acc = 0.0;
     for (int r = 0; r < D->n_targets; ++r) {
       const int nj = D->n_sd_next[r];
       const T* __restrict__ Mr = static_cast<const T*>(D->M[r]) + (int64_t(sd) * n_d + d) * nj;
       const T* __restrict__ Vr = static_cast<const T*>(D->Vn[r]);
       for (int j = 0; j < nj; ++j) {
         const T m = Mr[j];
-        if (m != T(0)) acc += m * Vr[int64_t(j) * g_n + k];
+        if (m != T(0)) acc = __builtin_fma(double(m), double(Vr[int64_t(j) * g_n + k]), acc);
       }
     }
-    out[k] = acc;
-    nonfinite |= !is_finite(acc);
+    const T v = T(acc);
+    out[k] = v;
+    nonfinite |= !is_finite(v);
   }
   const bool row_finite = __syncthreads_or(nonfinite) == 0;
   if (threadIdx.x == 0 && E.fin) E.fin[row] = row_finite ? 1 : 0;

@@ -84,7 +84,7 @@ def test_mortality_model_full_size(native, dtype):
         m, p = ex.mortality_model(T), ex.mortality_params(T)
         V, P = m.solve(params=p, log_level="off", dtype=dtype, return_policy=True)
         Vo, Po, Qo = og.solve(m, p, grid_dtype=dtype, return_Q=True)
-        worst, mism, ties = compare_solution(V, P, Vo, Po, Qo, dtype)
+        worst, mism, ties = compare_solution(V, P, Vo, Po, Qo, dtype, model=m, params=p)
         print(f"T={T} {np.dtype(dtype)}: max rel V err {worst:.3e}, mismatches {mism}, near-ties {ties}")
         assert worst <= (1e-10 if dtype == np.float64 else 1e-5)
         assert mism == 0

@@ -2,7 +2,10 @@
 
 Bars: float64 V within 1e-10 relative, float32 V within 1e-5 relative of the
 oracle (which solves the same float32-rounded grids in float64); policies
-identical except flagged near-ties (oracle Q within 8 ulp of the dtype).
+identical except flagged near-ties: oracle Q within one ulp of the dtype, or
+up to 8 ulp where Q in the reference's own arithmetic at that precision
+(from the GPU's V_{t+1}) also departs from the float64 argmax or agrees with
+the GPU (`tests/parity.py`).
 """
 
 import numpy as np
@@ -20,8 +23,9 @@ RTOL = {np.float64: 1e-10, np.float32: 1e-5}
 def _check(model, params, dtype, taste_regimes=()):
     V, P = model.solve(params=params, log_level="off", dtype=dtype, return_policy=True)
     Vo, Po, Qo = og.solve(model, params, grid_dtype=dtype, return_Q=True)
-    worst, mism, ties = compare_solution(V, P, Vo, Po, Qo, dtype, taste_regimes)
-    print(f"max rel V err {worst:.3e}, policy mismatches {mism}, flagged near-ties {ties}")
+    worst, mism, ties, split = compare_solution(V, P, Vo, Po, Qo, dtype, taste_regimes, model=model,
+                                                params=params, detail=True)
+    print(f"max rel V err {worst:.3e}, policy mismatches {mism}, flagged near-ties {ties} {split}")
     assert worst <= RTOL[dtype]
     assert mism == 0
     return V, P

@@ -264,3 +264,41 @@ def test_oracle_against_analytical_solution(case, n_periods, disutility):
         num = np.stack([V[t][reg] for t in range(n_periods - 1)])
         mse = np.mean((ana - num) ** 2, axis=0)
         assert np.all(np.abs(mse[2:]) < 1.5 * 10**-1)
+
+
+# ------------------------------------------- Q in the reference's arithmetic
+
+
+@pytest.mark.parametrize("case", ["stochastic", "mortality", "processes", "log_grid"])
+def test_reference_form_q_in_float64_is_the_oracle_q(case):
+    """`reference_form_q` at float64 reproduces the oracle's Q bit for bit at
+    every state (same order of operations), so at float32 it differs only by
+    the rounding of each step."""
+    from tests import models as tm
+
+    if case == "stochastic":
+        m = ex.retirement_model(4, n_wealth=30, n_consumption=40, income_nodes=5)
+        p = ex.retirement_params(4, income={"rho": 0.9, "sigma": 0.2, "mu": 0.0})
+    elif case == "mortality":
+        m, p = ex.mortality_model(4, n_wealth=20, n_consumption=30), ex.mortality_params(4)
+    elif case == "processes":
+        m, p = tm.processes_model(4, "tauchen"), tm.reference_processes_params("tauchen")
+    else:
+        m = tm.log_grid_model(4)
+        p = ex.retirement_params(4, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0})
+    V, _, QS = og.solve(m, p, return_Q=True)
+    names = list(m.regime_id_class.__categories__)
+    R = {n: og._R(n, m.regimes[n], np.dtype(np.float64), p, m.fixed_params) for n in names}
+    for (t, r), (Q, F) in QS.items():
+        reg = R[r]
+        tab = og.tabulate(m, R, reg, t, p, m.fixed_params, np.dtype(np.float64), V.get(t + 1))
+        n_states = int(np.prod(reg.V_shape)) if reg.V_shape else 1
+        states = np.arange(n_states)
+        Q64, F64 = og.reference_form_q(tab, reg, states, compute_dtype=np.float64)
+        k = n_states
+        assert np.array_equal(Q64.reshape(k, -1), Q.reshape(k, -1), equal_nan=True), (t, r)
+        assert np.array_equal(F64.reshape(k, -1), F.reshape(k, -1))
+        Q32, _ = og.reference_form_q(tab, reg, states, compute_dtype=np.float32)
+        fin = np.isfinite(Q64)
+        assert Q32.dtype == np.float32
+        np.testing.assert_allclose(Q32[fin].astype(np.float64), Q64[fin], rtol=1e-5, atol=1e-6)
