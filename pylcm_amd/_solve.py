@@ -20,15 +20,21 @@ def _has_dcegm(model):
 
 
 def solve_model(model, params, dtype):
-    """One solve: returns (period->regime->V, period->regime->policy) on the host."""
+    """One solve: (period->regime->V, period->regime->policy,
+    period->regime->published simulation policy) on the host. The policy is
+    the flat argmax index for grid-search regimes; DC-EGM regimes publish
+    their refined (grid, consumption) rows, grid-search regimes none."""
     if _has_dcegm(model):
         from pylcm_amd._egm_program import solve_dcegm_model
 
-        return solve_dcegm_model(model, params, dtype)
+        V, P = solve_dcegm_model(model, params, dtype)
+        sim = {t: {r: {"grid": g, "policy": c} for r, (g, c) in regs.items()} for t, regs in P.items()}
+        return V, P, sim
     prog = GsProgram(model, [params], dtype)
     prog.run(use_graph=False)
     prog.fetch()
-    return _split(prog.results(0))
+    V, P = _split(prog.results(0))
+    return V, P, {}
 
 
 def solve_batch(model, params_list, dtype, *, tile=None, use_graph=True, return_policy=False):

@@ -16,8 +16,9 @@ class ModelInitializationError(PyLCMError):
     """The model specification is invalid."""
 
 
-class RegimeInitializationError(PyLCMError):
-    """A regime specification is invalid."""
+class RegimeInitializationError(ModelInitializationError):
+    """A regime specification is invalid (a model initialization error, as in
+    `src/lcm/exceptions.py:63`)."""
 
 
 class GridInitializationError(PyLCMError):

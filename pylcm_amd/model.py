@@ -92,21 +92,27 @@ class Model:
         return [r for r in self.regime_names if self.layouts[r].regime.active(age)]
 
     # ------------------------------------------------------------------ solve
-    def solve(self, *, params, log_level: str = "off", max_compilation_workers=None,
+    def solve(self, *, params, log_level: str, max_compilation_workers=None,
               log_path=None, log_keep_n_latest: int = 3, return_simulation_policy: bool = False,
               dtype=None, return_policy: bool = False):
         """Backward induction on the GPU (`src/lcm/model.py:440-519`).
 
-        Returns `period -> regime -> V`. With `return_policy=True` also returns
-        `period -> regime -> policy` (flat argmax index over the regime's
-        action axes, discrete actions first).
+        Returns `period -> regime -> V`. `log_level` ("off", "warning",
+        "progress", "debug") is required, as in the reference. With
+        `return_simulation_policy=True` also returns the solvers' published
+        simulation policies, `period -> regime -> {"grid", "policy"}`: the
+        refined endogenous grid and consumption rows of each DC-EGM regime
+        (shape `[discrete states..., discrete actions, n_pad]`); grid-search
+        regimes publish none and have no entry. With `return_policy=True`
+        (an extension) also returns `period -> regime -> policy`, the flat
+        argmax index over the regime's action axes, discrete actions first.
         """
         if log_level not in _LOG_LEVELS:
             raise ValueError(f"log_level must be one of {_LOG_LEVELS}")
         from pylcm_amd._solve import solve_model
 
         dt = np.dtype(dtype) if dtype is not None else config.canonical_float_dtype()
-        V, pol = solve_model(self, params, dt)
+        V, pol, sim_pol = solve_model(self, params, dt)
         try:
             self._check_nan(V, log_level)
         except InvalidValueFunctionError as exc:
@@ -128,10 +134,12 @@ class Model:
 
             save_solve_snapshot(model=self, params=params, period_to_regime_to_V_arr=Vm,
                                 log_path=log_path, log_keep_n_latest=log_keep_n_latest)
-        if return_policy or return_simulation_policy:
-            Pm = MappingProxyType({t: MappingProxyType(pol[t]) for t in sorted(pol)})
-            return Vm, Pm
-        return Vm
+        out = (Vm,)
+        if return_simulation_policy:
+            out += (MappingProxyType({t: MappingProxyType(sim_pol.get(t, {})) for t in sorted(V)}),)
+        if return_policy:
+            out += (MappingProxyType({t: MappingProxyType(pol[t]) for t in sorted(pol)}),)
+        return out if len(out) > 1 else Vm
 
     def _check_nan(self, V, log_level):
         if log_level == "off":
@@ -157,17 +165,22 @@ class Model:
                            return_policy=return_policy)
 
     # --------------------------------------------------------------- simulate
-    def simulate(self, *, params, initial_conditions, period_to_regime_to_V_arr=None,
-                 log_level: str = "off", seed: int | None = None, subject_batch_size: int = 0,
+    def simulate(self, *, params, initial_conditions, period_to_regime_to_V_arr,
+                 log_level: str, seed: int | None = None, subject_batch_size: int = 0,
                  log_path=None, log_keep_n_latest: int = 3, max_compilation_workers=None,
                  dtype=None):
-        """Forward simulation on the GPU (`src/lcm/model.py:648-848`)."""
+        """Forward simulation on the GPU (`src/lcm/model.py:648-848`).
+
+        `period_to_regime_to_V_arr` (None: solve first) and `log_level` are
+        required, as in the reference."""
+        if log_level not in _LOG_LEVELS:
+            raise ValueError(f"log_level must be one of {_LOG_LEVELS}")
         from pylcm_amd._simulate import simulate_model
 
         dt = np.dtype(dtype) if dtype is not None else config.canonical_float_dtype()
         snapshot = log_path is not None and log_level == "debug"
         if snapshot and period_to_regime_to_V_arr is None and not self._any_dcegm():
-            period_to_regime_to_V_arr = self.solve(params=params, dtype=dt)
+            period_to_regime_to_V_arr = self.solve(params=params, log_level="off", dtype=dt)
         result = simulate_model(self, params=params, initial_conditions=initial_conditions,
                                 V=period_to_regime_to_V_arr, seed=seed, dtype=dt,
                                 subject_batch_size=subject_batch_size, log_level=log_level)

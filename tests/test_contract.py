@@ -48,3 +48,21 @@ def test_fixed_params_bind_into_the_core_and_lower_args():
                               next_regime_to_continuation=MappingProxyType({}), flat_params={}, period=1, ages=m.ages)
     assert {"wealth", "labor_supply", "consumption", "next_regime_to_V_arr", "period", "age"} <= set(args)
     assert np.allclose(args["wealth"][[0, -1]], [1.0, 400.0]) and args["age"] == 1
+
+
+def test_api_keywords_and_exception_hierarchy():
+    """`log_level` is required by solve and simulate, `period_to_regime_to_V_arr`
+    by simulate (`src/lcm/model.py:440-449,648-660`); a regime error is a
+    model initialization error (`src/lcm/exceptions.py:59-70`)."""
+    import pytest
+
+    from pylcm_amd.exceptions import ModelInitializationError, RegimeInitializationError
+
+    m, p = ex.deterministic_workload()
+    with pytest.raises(TypeError):
+        m.solve(params=p)
+    with pytest.raises(TypeError):
+        m.simulate(params=p, initial_conditions={}, log_level="off")
+    with pytest.raises(ValueError):
+        m.simulate(params=p, initial_conditions={}, period_to_regime_to_V_arr=None, log_level="loud")
+    assert issubclass(RegimeInitializationError, ModelInitializationError)

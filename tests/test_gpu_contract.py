@@ -96,3 +96,21 @@ def test_contract_loop_nan_fail_fast(native):
         bi.solve(m, flat_params=tm.mortality_mass_params(T, mass_scale=1.01), log_level="debug",
                  dtype=np.float64)
     assert set(info.value.partial_solution) == {T - 1}
+
+
+def test_return_simulation_policy(native):
+    """DC-EGM regimes publish their refined (grid, consumption) rows as the
+    simulation policy; grid-search regimes publish none (`src/lcm/model.py:484-497`)."""
+    m = ex.iskhakov_dcegm_model(4)
+    p = ex.retirement_params(4, discount_factor=0.98, disutility_of_work=1.0, interest_rate=0.0, wage=20.0,
+                             log_utility=True)
+    V, pol = m.solve(params=p, log_level="off", dtype=np.float64, return_simulation_policy=True)
+    assert set(pol) == set(V)
+    assert set(pol[0]) == {"working_life", "retirement"} and not pol[3]
+    g, c = pol[0]["working_life"]["grid"], pol[0]["working_life"]["policy"]
+    assert g.shape == c.shape and g.shape[-2] == 2
+    ok = np.isfinite(g) & np.isfinite(c)
+    assert ok.any() and np.all(c[ok] > 0)
+    gs = ex.retirement_model(4, n_wealth=20, n_consumption=30)
+    V2, pol2 = gs.solve(params=ex.retirement_params(4), log_level="off", return_simulation_policy=True)
+    assert set(pol2) == set(V2) and all(len(pol2[t]) == 0 for t in pol2)

@@ -56,7 +56,7 @@ def test_reference_simulation_frames(native, name, prec):
     model, params = rf.model_and_params(name)
     rows, ic = rf.visited_states(frame, model)
     V = _solve(model, params, dtype)
-    df = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V,
+    df = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off",
                         seed=5, dtype=dtype).to_dataframe(terminal_rows="all")
     start = np.array([model.ages.age_to_period(a) for a in ic["age"]])
     n = len(rows)
@@ -122,7 +122,8 @@ def test_simulated_survival_frequencies(native):
     m, p = ex.mortality_model(T, n_wealth=60, n_consumption=120), ex.mortality_params(T)
     n = 200_000
     ic = {"age": np.full(n, 40.0), "wealth": np.full(n, 100.0), "regime_id": np.zeros(n, dtype=np.int32)}
-    df = m.simulate(params=p, initial_conditions=ic, seed=11, dtype=np.float64).to_dataframe(terminal_rows="all")
+    df = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=None, log_level="off", seed=11,
+                    dtype=np.float64).to_dataframe(terminal_rows="all")
     sp = ex.default_survival_probs(T)
     alive = ~df["regime_name"].astype(str).isin(["dead"])
     expected = 1.0

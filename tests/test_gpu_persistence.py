@@ -32,13 +32,13 @@ def _same_V(a, b):
 
 def test_saved_solution_warm_starts_simulate(native, tmp_path):
     m, p, ic = _model()
-    V = m.solve(params=p, dtype=np.float64)
+    V = m.solve(params=p, log_level="off", dtype=np.float64)
     path = save_solution(period_to_regime_to_V_arr=V, path=tmp_path / "solution.h5")
     Vl = load_solution(path=path)
     _same_V(V, Vl)
-    a = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=4,
+    a = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off", seed=4,
                    dtype=np.float64).raw
-    b = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=Vl, seed=4,
+    b = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=Vl, log_level="off", seed=4,
                    dtype=np.float64).raw
     for k in ("action", "regime", "value", "w"):
         assert np.array_equal(a[k], b[k], equal_nan=True), k
@@ -53,7 +53,7 @@ def test_debug_solve_and_simulate_write_snapshots(native, tmp_path):
     assert isinstance(s, SolveSnapshot)
     _same_V(V, s.period_to_regime_to_V_arr)
     # the unpickled model solves to the same V
-    _same_V(V, s.model.solve(params=s.params, dtype=np.float64))
+    _same_V(V, s.model.solve(params=s.params, log_level="off", dtype=np.float64))
 
     res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=2,
                      dtype=np.float64, log_level="debug", log_path=tmp_path)

@@ -37,7 +37,7 @@ def test_device_philox_matches_oracle(native):
 def _compare(model, params, ic, dtype, seed, rtol, V=None, oracle_model=None):
     if V is None:
         V, _ = og.solve(model, params, grid_dtype=dtype)
-    res = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V,
+    res = model.simulate(params=params, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off",
                          seed=seed, dtype=dtype)
     got = res.raw
     model = oracle_model or model
@@ -164,7 +164,8 @@ def test_simulate_refuses_state_dependent_utility(native):
     p = ex.retirement_params(4, log_utility=True)
     ic = {"wealth": np.array([10.0, 50.0]), "regime_id": np.zeros(2, dtype=int), "age": np.zeros(2)}
     with pytest.raises(UnsupportedModelError):
-        m.simulate(params=p, initial_conditions=ic, seed=1, dtype=np.float64)
+        m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=None, log_level="off", seed=1,
+                   dtype=np.float64)
 
 
 def test_simulate_large_panel_f32(native):
@@ -182,7 +183,7 @@ def test_simulate_large_panel_f32(native):
     w0 = og.linspace(1, 400, 100, np.float32)[rng.integers(0, 100, n)]
     ic = {"wealth": w0, "income": z0, "regime_id": np.zeros(n, dtype=int)}
     V, _ = m.solve(params=p, log_level="off", dtype=np.float32, return_policy=True)
-    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=2026,
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off", seed=2026,
                      dtype=np.float32)
     raw = res.raw
     assert np.all(raw["action"][0] >= 0)
@@ -224,7 +225,8 @@ def test_simulate_taste_shock_choice_frequencies_f32(native):
     w0 = np.full(n, 37.0)
     z0 = np.zeros(n)
     ic = {"wealth": w0, "income": z0, "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
-    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, seed=3, dtype=np.float32)
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off", seed=3,
+                     dtype=np.float32)
     d = res.raw["action"][0] // 40
     freq = np.array([np.mean(d == 0), np.mean(d == 1)])
     one = {k: v[:1] for k, v in ic.items()}

@@ -164,4 +164,4 @@ def test_solve_without_a_device_raises_instead_of_falling_back():
         pytest.skip("a GPU is present")
     m, p = ex.deterministic_workload()
     with pytest.raises(NativeKernelError):
-        m.solve(params=p)
+        m.solve(params=p, log_level="off")
