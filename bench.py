@@ -369,6 +369,30 @@ def serving_leg(args):
     }
 
 
+def fresh_leg(args, models, T, tile):
+    """Fresh draws: every timed batch holds NEW parameter draws of the batched
+    workload, lowered and packed on the host by worker processes and placed
+    into pinned host memory (batch k+1 while the GPU solves batch k), then
+    uploaded as one image and solved with one fused launch per period."""
+    from pylcm_amd._batch import FreshBatches
+
+    n = args.fresh_batches + 1  # the first batch fills the pipeline
+    batches = [[(s, p) for s, p in __import__("pylcm_amd.examples", fromlist=["x"]).draw_batch_params(
+        args.draws, n_periods=T, seed=5000 + k)] for k in range(n)]
+    fb = FreshBatches(models, np.float32, procs=args.fresh_procs, tile=tile, chunk=16)
+    stamps = []
+    try:
+        stats = fb.run(batches, on_done=lambda k, ab: stamps.append(time.perf_counter()))
+    finally:
+        fb.close()
+    rate = args.draws * (n - 1) / (stamps[-1] - stamps[0])
+    med = {k: round(float(np.median([st[k] for st in stats[1:]])), 4) for k in stats[0]}
+    log(f"fresh draws: {rate:.1f} solves/s over {n - 1} batches of {args.draws} "
+        f"({args.fresh_procs} lowering processes); per batch median {med}")
+    return {"solves_per_s": round(rate, 2), "batches": n - 1, "draws_per_batch": args.draws,
+            "lowering_processes": args.fresh_procs, "per_batch_median_s": med}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -387,13 +411,15 @@ def main():
     ap.add_argument("--no-sim", action="store_true")
     ap.add_argument("--sim-agents", type=int, default=100_000)
     ap.add_argument("--sim-cpu-agents", type=int, default=1500)
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-sat-rate", type=float, default=1000.0)
     ap.add_argument("--serve-sat-seconds", type=float, default=3.0)
-    ap.add_argument("--serve-procs", type=int, default=8)
+    ap.add_argument("--serve-procs", type=int, default=12)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
     ap.add_argument("--serve-tile", type=int, default=0, help="0: chosen per fused launch")
+    ap.add_argument("--fresh-batches", type=int, default=4)
+    ap.add_argument("--fresh-procs", type=int, default=14)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -546,6 +572,7 @@ def main():
                   **grading, "checked_draws": len(order)}
         log(f"cpu baseline: {cpu['value']:.3f} solves/s on {threads} threads; parity {parity}")
 
+    fresh = fresh_leg(args, models, T, prog.tile) if args.fresh_batches > 0 else None
     dcegm = None if args.no_dcegm else dcegm_leg(args)
     simulation = None if args.no_sim else simulation_leg(args)
     serving = serving_leg(args) if args.serve_seconds > 0 else None
@@ -579,6 +606,10 @@ def main():
             "sample": n_plain,
         },
         "batched_vs_plain": round(value / plain_sps, 2),
+        "value_fresh_draws": fresh["solves_per_s"] if fresh else None,
+        "fresh_draws": fresh,
+        "fresh_vs_plain_end_to_end": (round(fresh["solves_per_s"] * float(np.percentile(e2e, 50)), 2)
+                                      if fresh else None),
         "batched_latency_ms": round(ms_per_step, 3),
         "roofline": {
             "bound": "valu",

@@ -491,8 +491,7 @@ class PeriodLowering:
         ids_all = self._ids_all_periods(lay, tr)
         if ids_all is None:
             return None
-        age_next = self.model.ages.period_to_age(period + 1)
-        active_next = tuple(r for r in self.model.regime_names if self.model.layouts[r].regime.active(age_next))
+        active_next = tuple(self.model.active_regimes(period + 1))
         return ("period", lay.name, active_next, ids_all[period].tobytes())
 
     def regime_probs(self, lay, period):
@@ -594,11 +593,10 @@ class PeriodLowering:
             pt.taste_scale = float(ts["scale"])
             if not pt.taste_scale > 0:
                 raise InvalidParamsError("taste shock scale must be positive")
-        age_next = model.ages.period_to_age(period + 1)
         tr = reg.transition
         tr_funcs = list(tr.values()) if hasattr(tr, "items") else [tr]
         p = self._cached("P", lay, tr_funcs, period, lambda: self.regime_probs(lay, period))
-        active_next = [r for r in model.regime_names if model.layouts[r].regime.active(age_next)]
+        active_next = model.active_regimes(period + 1)
         for r in model.regime_names:
             code = model.regime_code(r)
             if r not in active_next and np.any(p[:, :, code] != 0):

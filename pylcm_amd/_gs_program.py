@@ -39,12 +39,16 @@ def _align(x: int) -> int:
 class _Arena:
     """Host image of a device arena with content-deduplicated tables."""
 
-    def __init__(self):
+    def __init__(self, digest: bool = True):
         self.parts: list[tuple[int, bytes]] = []
         self.size = 0
         self.index: dict[str, int] = {}
         self.by_id: dict[tuple, int] = {}
         self._keep: list = []
+        # content digests find equal tables that are different objects (across
+        # the problems of a batch); within one problem's lowering equal tables
+        # are already the same object
+        self.digest = digest
 
     def add(self, arr: np.ndarray, dedupe: bool = True, dtype=None) -> int:
         """Place `arr` (cast to `dtype`) in the arena; the same array object or
@@ -54,7 +58,7 @@ class _Arena:
             if idk in self.by_id:
                 return self.by_id[idk]
         a = np.ascontiguousarray(arr if dtype is None else arr.astype(dtype))
-        key = table_digest(a) + str(a.dtype) if dedupe else None
+        key = table_digest(a) + str(a.dtype) if dedupe and self.digest else None
         if key is not None and key in self.index:
             off = self.index[key]
         else:
@@ -136,9 +140,10 @@ def chunk_bounds(lo, hi, chunk):
     states all lie outside [min lo, max hi] skips the chunk."""
     n_sdx, _, n_d, n_c = lo.shape
     n_ch = (n_c + chunk - 1) // chunk
-    pad = n_ch * chunk - n_c
-    lo_p = np.pad(lo[:, 0], ((0, 0), (0, 0), (0, pad)), constant_values=np.inf)
-    hi_p = np.pad(hi[:, 0], ((0, 0), (0, 0), (0, pad)), constant_values=-np.inf)
+    lo_p = np.full((n_sdx, n_d, n_ch * chunk), np.inf, dtype=lo.dtype)
+    hi_p = np.full((n_sdx, n_d, n_ch * chunk), -np.inf, dtype=hi.dtype)
+    lo_p[:, :, :n_c] = lo[:, 0]
+    hi_p[:, :, :n_c] = hi[:, 0]
     out = np.empty((n_sdx, n_d, n_ch, 2), dtype=lo.dtype)
     out[..., 0] = lo_p.reshape(n_sdx, n_d, n_ch, chunk).min(axis=-1)
     out[..., 1] = hi_p.reshape(n_sdx, n_d, n_ch, chunk).max(axis=-1)
@@ -176,11 +181,8 @@ def lower_problem(model, params, dtype):
     tabs = {}
     same = {}  # period key -> tables shared by every period with that key
     for t in range(model.n_periods):
-        age = model.ages.period_to_age(t)
-        for r in model.regime_names:
+        for r in model.active_regimes(t):
             lay = model.layouts[r]
-            if not lay.regime.active(age):
-                continue
             if lay.is_dcegm:
                 raise UnsupportedModelError("DC-EGM regimes are solved by the DC-EGM program")
             key = low.period_key(lay, t)
@@ -262,7 +264,7 @@ class GsProgram:
         dt = self.dt
         isz = dt.itemsize
         S = self.n_steps
-        ar = _Arena()
+        ar = _Arena(digest=len(self.problems) > 1)
         self.outs: list[dict] = []
         # outputs are reserved first so that descriptors can point at them
         for model, tabs in zip(self.models, self.problems):
@@ -278,8 +280,11 @@ class GsProgram:
         self.out_end = ar.size
         self.out_start = _align(self.out_end)  # every table is placed from here on
         zero_off = ar.add(np.zeros(1, dtype=dt))
-        descs = []
-        tasks = [[] for _ in range(S)]
+        # descriptors: few distinct templates (tables, sizes) and per
+        # descriptor only the output and next-period V offsets
+        self._templates = []
+        tpl_of, v_out, p_out, vn, vs = [], [], [], [], []
+        tasks = [[] for _ in range(S)]  # step -> descriptor indices, in descriptor order
         wdep = [0] * S  # kernel variant per step: the most general any descriptor needs
         gmax = [1] * S
         kmax = [0] * S
@@ -295,21 +300,32 @@ class GsProgram:
             for (t, r), pt in order:
                 s = self.step_of(p, t)
                 lay = model.layouts[r]
-                d = self._describe_cached(model, p, t, r, pt, rg, ar, zero_off, dcache)
-                di = len(descs)
-                descs.append(d)
+                ti, patch = self._describe_cached(model, p, t, r, pt, rg, ar, zero_off, dcache)
+                d = self._templates[ti]
+                di = len(tpl_of)
+                tpl_of.append(ti)
+                v_out.append(patch[0])
+                p_out.append(patch[1])
+                vn.append(patch[2])
+                vs.append(patch[3])
                 self.desc_of[(p, t, r)] = di
-                wdep[s] = max(wdep[s], d["variant"])
-                kmax[s] = max(kmax[s], d["n_constr"])
-                if d["g_n"] > 0:
-                    gmax[s] = max(gmax[s], d["g_n"])
-                n_states = lay.n_sd * lay.n_w
-                n_q = n_states * lay.n_d * lay.n_c
-                ev_flops = 2.0 * lay.n_sd * lay.n_d * sum(d["nj"]) * max(d["g_n"], 0)
-                self.flops_by_step[s] += FLOPS_PER_Q * n_q + ev_flops
-                # algorithmic HBM bytes: next V rows read, V and policy written
-                n_next = sum(nj * d["g_n"] for nj in d["nj"])
-                self.bytes_by_step[s] += isz * (n_next + n_states) + 4 * n_states
+                if d["variant"] > wdep[s]:
+                    wdep[s] = d["variant"]
+                if d["n_constr"] > kmax[s]:
+                    kmax[s] = d["n_constr"]
+                if d["g_n"] > gmax[s]:
+                    gmax[s] = d["g_n"]
+                acct = d.get("_acct")
+                if acct is None:
+                    n_states = lay.n_sd * lay.n_w
+                    n_q = n_states * lay.n_d * lay.n_c
+                    ev_flops = 2.0 * lay.n_sd * lay.n_d * sum(d["nj"]) * max(d["g_n"], 0)
+                    # algorithmic HBM bytes: next V rows read, V and policy written
+                    n_next = sum(nj * d["g_n"] for nj in d["nj"])
+                    acct = d["_acct"] = (FLOPS_PER_Q * n_q + ev_flops,
+                                         isz * (n_next + n_states) + 4 * n_states)
+                self.flops_by_step[s] += acct[0]
+                self.bytes_by_step[s] += acct[1]
                 # tasks (descriptor, discrete state, first wealth node), sd-major
                 key = (lay.n_sd, lay.n_w)
                 grid = task_grids.get(key)
@@ -317,21 +333,30 @@ class GsProgram:
                     sd_i, w_i = np.meshgrid(np.arange(lay.n_sd, dtype=np.int32),
                                             np.arange(0, lay.n_w, self.tile, dtype=np.int32), indexing="ij")
                     grid = task_grids[key] = np.stack([sd_i.ravel(), w_i.ravel()], axis=1)
-                tasks[s].append((di, grid))
+                tasks[s].append(di)
                 self.desc_tasks[di] = grid
-        self.n_desc = len(descs)
+        self.n_desc = len(tpl_of)
+        self._tpl_of = np.asarray(tpl_of, dtype=np.int64)
+        self._patch = (np.asarray(v_out, dtype=np.uint64), np.asarray(p_out, dtype=np.uint64),
+                       np.asarray(vn, dtype=np.int64).reshape(self.n_desc, nat.MAX_TARGETS),
+                       np.asarray(vs, dtype=np.int64).reshape(self.n_desc, nat.MAX_SCALAR))
         self.desc_off = ar.reserve(self.n_desc * DESC_DTYPE.itemsize)
         self.task_off, self.n_tasks = [], []
         self.tasks_host = []
         for s in range(S):
-            parts = tasks[s]
-            n = sum(len(g) for _, g in parts)
-            tl_a = np.empty((n, 3), dtype=np.int32)
-            i = 0
-            for di, g in parts:
-                tl_a[i: i + len(g), 0] = di
-                tl_a[i: i + len(g), 1:] = g
-                i += len(g)
+            # descriptor order: each descriptor's tasks stay contiguous (they
+            # share its tables in L2) and problems of different sizes stay
+            # interleaved, so the contiguous task ranges the kernel hands to
+            # the eight XCDs carry similar work
+            dis = tasks[s]
+            if dis:
+                grids = [self.desc_tasks[di] for di in dis]
+                tl_a = np.empty((sum(len(g) for g in grids), 3), dtype=np.int32)
+                tl_a[:, 0] = np.repeat(np.asarray(dis, dtype=np.int32), [len(g) for g in grids])
+                tl_a[:, 1:] = np.concatenate(grids)
+            else:
+                tl_a = np.empty((0, 3), dtype=np.int32)
+            n = len(tl_a)
             self.tasks_host.append(tl_a)
             if not n:
                 self.task_off.append(0)
@@ -349,7 +374,7 @@ class GsProgram:
         self._lds_g, self._lds_k = gmax, kmax
         self._lds_by_step = None
         self.total_bytes = _align(ar.size)
-        self._arena, self._descs = ar, descs
+        self._arena = ar
         self.out_host = np.zeros(self.out_end, dtype=np.uint8)
         self.buf = None
         if not self._defer:
@@ -368,7 +393,7 @@ class GsProgram:
         """Write the descriptors for device address `ptr` and upload the image
         there (a synchronous copy on `upload_ctx`, default the program's stream)."""
         self.base = int(ptr)
-        self.D = self._desc_records(self._descs)
+        self.D = self._desc_records()
         ar = self._arena
         ar.parts.append((self.desc_off, self.D.tobytes()))
         # the V and policy outputs come first and are written by the kernels
@@ -388,16 +413,17 @@ class GsProgram:
         hit = cache.get(key)
         if hit is None:
             d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
-            cache[key] = (d, pt)  # pt keeps the keyed tables alive
-            return d
-        d = dict(hit[0])
+            self._templates.append(d)
+            hit = cache[key] = (len(self._templates) - 1, pt)  # pt keeps the keyed tables alive
         out = self.outs[p][(t, r)]
-        d["V_out"], d["pol_out"] = out.v_off, out.p_off
+        vn = [-1] * nat.MAX_TARGETS
+        vs = [-1] * nat.MAX_SCALAR
         if not pt.terminal:
-            d["Vn"] = [self.outs[p][(t + 1, name)].v_off for name, _ in pt.targets]
-            d["Vs"] = [self.outs[p][(t + 1, name)].v_off if name is not None else zero_off
-                       for name, _ in pt.scalars]
-        return d
+            for k, (name, _) in enumerate(pt.targets):
+                vn[k] = self.outs[p][(t + 1, name)].v_off
+            for k, (name, _) in enumerate(pt.scalars):
+                vs[k] = self.outs[p][(t + 1, name)].v_off if name is not None else zero_off
+        return hit[0], (out.v_off, out.p_off, vn, vs)
 
     def _describe(self, model, p, t, r, pt, rg, ar, zero_off):
         dt = self.dt
@@ -491,7 +517,7 @@ class GsProgram:
     def relocatable_records(self):
         """(records for base address 0, {field: mask of its pointer entries}):
         `records_at(base)` turns them into the descriptors for any base."""
-        return self._desc_records(self._descs, base=0, with_masks=True)
+        return self._desc_records(base=0, with_masks=True)
 
     @staticmethod
     def records_at(D0, masks, base: int):
@@ -512,18 +538,35 @@ class GsProgram:
                     ("n_sd_next", "nj", nat.MAX_TARGETS, False, None),
                     ("Vs", "Vs", nat.MAX_SCALAR, True, None), ("ms", "ms", nat.MAX_SCALAR, True, None))
 
-    def _desc_records(self, descs, base=None, with_masks=False):
-        """Descriptor records, filled column by column; pointers are arena
-        offsets plus `base` (default: the program's). With `with_masks`, also
-        returns {field: mask of its pointer entries}."""
+    def _desc_records(self, base=None, with_masks=False):
+        """Descriptor records: the templates' records (filled column by
+        column), gathered per descriptor, with each descriptor's output and
+        next-period V offsets patched in; pointers are arena offsets plus
+        `base` (default: the program's). With `with_masks`, also returns
+        {field: mask of its pointer entries}."""
         base = self.base if base is None else base
+        T, tmasks = self._template_records(base)
+        D = T[self._tpl_of]
+        v_out, p_out, vn, vs = self._patch
+        D["V_out"] = v_out + np.uint64(base)
+        D["pol_out"] = p_out + np.uint64(base)
+        for field, arr in (("Vn", vn), ("Vs", vs)):
+            present = arr >= 0
+            if present.any():
+                D[field] = np.where(present, arr.astype(np.uint64) + np.uint64(base), D[field]).astype(np.uint64)
+        if not with_masks:
+            return D
+        return D, {f: m[self._tpl_of] for f, m in tmasks.items()}
+
+    def _template_records(self, base):
+        """Records of the descriptor templates and {field: pointer mask}."""
+        descs = self._templates
         n = len(descs)
         D = np.zeros(n, dtype=DESC_DTYPE)
         masks = {}
         for f in self._PTR_FIELDS:
             D[f] = np.uint64(base) + np.fromiter((d[f] for d in descs), dtype=np.uint64, count=n)
-            if with_masks:
-                masks[f] = np.ones(n, dtype=bool)
+            masks[f] = np.ones(n, dtype=bool)
         for f in self._VAL_FIELDS:
             D[f] = np.array([d[f] for d in descs], dtype=D.dtype[f])
         D["n_targets"] = np.array([len(d["Vn"]) for d in descs], dtype=np.int32)
@@ -541,10 +584,9 @@ class GsProgram:
             present = np.arange(k)[None, :] < lens[:, None]
             if ptr:
                 vals = np.where(present, vals + np.uint64(base), vals).astype(np.uint64)
-                if with_masks:
-                    masks[field] = present
+                masks[field] = present
             D[field] = vals
-        return (D, masks) if with_masks else D
+        return D, masks
 
     # -- execution -------------------------------------------------------------
     def _launch_step(self, s: int):

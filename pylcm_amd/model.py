@@ -88,8 +88,16 @@ class Model:
         return self.regime_names_to_ids[name]
 
     def active_regimes(self, period: int):
-        age = self.ages.period_to_age(period)
-        return [r for r in self.regime_names if self.layouts[r].regime.active(age)]
+        """Regimes active in `period` (evaluated once per model, then cached)."""
+        cache = self.__dict__.setdefault("_active_cache", {})
+        hit = cache.get(period)
+        if hit is None:
+            age = self.ages.period_to_age(period)
+            hit = cache[period] = tuple(r for r in self.regime_names if self.layouts[r].regime.active(age))
+        return list(hit)
+
+    def is_active(self, regime: str, period: int) -> bool:
+        return regime in self.active_regimes(period)
 
     # ------------------------------------------------------------------ solve
     def solve(self, *, params, log_level: str, max_compilation_workers=None,

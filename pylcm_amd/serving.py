@@ -39,15 +39,10 @@ def _lower_in_worker(blob: bytes, dtype: str, tile: int) -> dict:
     loaded there): everything the server needs to place and run it."""
     import cloudpickle
 
+    from pylcm_amd._batch import pack_problem
+
     model, params = cloudpickle.loads(blob)
-    dt = np.dtype(dtype)
-    low = lower_problem(model, params, dt)
-    prog = GsProgram.of([(model, params)], dt, lowered=[low], tile=tile, defer=True)
-    D0, masks = prog.relocatable_records()
-    return {"image": prog._arena.image(prog.out_start), "D0": D0, "masks": masks,
-            "tasks_host": prog.tasks_host, "outs": prog.outs, "n_steps": prog.n_steps,
-            "lds_g": prog._lds_g, "lds_k": prog._lds_k, "wdep_by_step": prog.wdep_by_step,
-            "total_bytes": prog.total_bytes, "out_start": prog.out_start, "n_desc": prog.n_desc}
+    return pack_problem(model, params, dtype, tile)
 
 
 def _warm_worker() -> int:
@@ -116,13 +111,10 @@ class Request:
     t_lowered: float = 0.0
     t_admit: float = 0.0
     t_done: float = 0.0
-    step: int = 0
     slab: tuple | None = None
     desc: tuple | None = None
     tasks: list | None = None
     desc_base: int = 0
-    n_w: object = None
-    done_event: object = None
     future: object = None
     result: dict | None = None
     done: threading.Event = field(default_factory=threading.Event)
@@ -130,6 +122,33 @@ class Request:
     @property
     def latency(self) -> float:
         return self.t_done - self.t_submit
+
+
+class _Cohort:
+    """Requests admitted together: their per-step task arrays concatenated
+    once (member p runs steps 0 .. n_steps_p - 1), so a fused launch costs one
+    array per cohort rather than one per request."""
+
+    def __init__(self, members: list):
+        self.members = members
+        self.n_steps = max(r.prog.n_steps for r in members)
+        self.steps, self.lds, self.variant = [], [], []
+        self._ending = {}
+        for r in members:
+            self._ending.setdefault(r.prog.n_steps - 1, []).append(r)
+        for s in range(self.n_steps):
+            parts = [r.tasks[s] for r in members if s < r.prog.n_steps and len(r.tasks[s])]
+            self.steps.append(np.concatenate(parts) if parts else np.zeros(0, dtype=TASK_DTYPE))
+            self.lds.append(max([r.prog.lds_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
+            self.variant.append(max([r.prog.wdep_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
+        self.s = 0
+
+    @property
+    def remaining(self) -> int:
+        return sum(1 for r in self.members if r.prog is not None and self.s < r.prog.n_steps)
+
+    def finished_at(self, s: int) -> list:
+        return self._ending.get(s, [])
 
 
 class SolveServer:
@@ -174,6 +193,7 @@ class SolveServer:
         self.desc_buf = nat.DeviceBuffer(self.desc_slab.size)
         self.max_tasks = int(max_tasks)
         self.max_inflight = int(max_inflight)
+        self.n_w_of_desc = np.zeros(int(max_desc), dtype=np.int32)  # for splitting tasks into fine tiles
         # at most two fused launches queued on the stream: while one runs, the
         # next one gathers every step that becomes ready, so launches stay large
         self.n_ring = 2
@@ -187,8 +207,8 @@ class SolveServer:
         self._ready_lock = threading.Lock()
         self._mem_cv = threading.Condition()  # guards the slabs; notified on retirement
         self._n_placed = 0
-        self.inflight: list[Request] = []
-        self.finishing: list[Request] = []
+        self.inflight: list[_Cohort] = []   # cohorts with steps left to launch
+        self.finishing: list = []            # (event, requests) in stream order
         self.completed: list[Request] = []
         self._next_id = 0
         self.launches = 0
@@ -299,10 +319,9 @@ class SolveServer:
             steps.append(rec)
         req.tasks = steps
         req.desc_base = desc_base
-        req.n_w = np.asarray(D0["n_w"], dtype=np.int64)  # per descriptor of the request
+        self.n_w_of_desc[desc_base: desc_base + prog.n_desc] = D0["n_w"]
         req.slab, req.desc = sl, ds
         req.t_admit = time.perf_counter()
-        req.step = 0
         with self._ready_lock:
             self._ready.append(req)
 
@@ -310,36 +329,45 @@ class SolveServer:
     def _admit(self):
         with self._ready_lock:
             ready, self._ready = self._ready, []
-        self.inflight.extend(ready)
+        if ready:
+            self.inflight.append(_Cohort(ready))
+
+    def _inflight_requests(self) -> int:
+        return sum(c.remaining for c in self.inflight)
 
     def _launch(self):
-        """One fused launch: the next non-empty backward step of every request."""
+        """One fused launch: the next backward step of every request in flight
+        (per cohort, one precomputed task array)."""
         k = self.ring_i
         if self.ring_used[k]:
             self.ring_events[k].synchronize()  # the copy from this host slot has finished
         rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
         picks, n_coarse, lds, variant = [], 0, 0, 0
-        for req in self.inflight:
-            prog = req.prog
-            while req.step < prog.n_steps and len(req.tasks[req.step]) == 0:
-                req.step += 1
-            if req.step >= prog.n_steps:
-                continue
-            tc = req.tasks[req.step]
-            if n_coarse + len(tc) + 8 > self.max_tasks:
+        for c in self.inflight:
+            arr = c.steps[c.s]
+            if n_coarse + len(arr) + 8 > self.max_tasks:
                 break  # the rest join the next launch
-            picks.append((req, req.step))
-            n_coarse += len(tc)
-            lds = max(lds, prog.lds_by_step[req.step])
-            variant = max(variant, prog.wdep_by_step[req.step])
-            req.step += 1
+            picks.append(c)
+            n_coarse += len(arr)
+            lds = max(lds, c.lds[c.s])
+            variant = max(variant, c.variant[c.s])
         if not picks:
             return False
+        if n_coarse == 0:  # nothing to run this step (no regime of these cohorts is active)
+            ev = nat.Event()
+            ev.record(self.ctx)
+            done = [r for c in picks for r in c.finished_at(c.s)]
+            if done:
+                self.finishing.append((ev, done))
+            for c in picks:
+                c.s += 1
+            self.inflight = [c for c in self.inflight if c.s < c.n_steps]
+            return True
         fine = self.auto_tile and n_coarse < self.fine_below and 4 * n_coarse + 8 <= self.max_tasks
         tile = self.FINE_TILE if fine else self.tile
         n = 0
-        for req, st in picks:
-            t = self._split(req, req.tasks[st]) if fine else req.tasks[st]
+        for c in picks:
+            t = self._split(self.n_w_of_desc, c.steps[c.s]) if fine else c.steps[c.s]
             rec[n: n + len(t)] = t
             n += len(t)
         self.tiles_used[tile] = self.tiles_used.get(tile, 0) + 1
@@ -359,48 +387,50 @@ class SolveServer:
         self.ring_used[k] = True
         self.launches += 1
         self.launch_tasks += n
-        still = []
-        for req in self.inflight:
-            if req.step >= req.prog.n_steps:
-                ev = nat.Event()
-                ev.record(self.ctx)
-                req.done_event = ev
-                self.finishing.append(req)
-            else:
-                still.append(req)
-        self.inflight = still
+        done = []
+        for c in picks:
+            done += c.finished_at(c.s)
+            c.s += 1
+        if done:
+            self.finishing.append((e_end, done))
+        self.inflight = [c for c in self.inflight if c.s < c.n_steps]
         return True
 
     @classmethod
-    def _split(cls, req, t):
+    def _split(cls, n_w_of_desc, t):
         """The 64-state tasks of 256-state tasks `t`: each tile's four
         quarters that start below its descriptor's n_w."""
         k = cls.COARSE_TILE // cls.FINE_TILE
         r = np.repeat(t, k)
         r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * cls.FINE_TILE, len(t))
-        return r[r["w0"] < req.n_w[r["desc"] - req.desc_base]]
+        return r[r["w0"] < n_w_of_desc[r["desc"]]]
 
     def _retire(self, block: bool = False):
-        still, freed = [], False
-        for req in self.finishing:
+        """Complete the requests whose last step has run. Events were recorded
+        on one stream, so they complete in order: polling stops at the first
+        one still pending."""
+        freed = False
+        while self.finishing:
+            ev, reqs = self.finishing[0]
             if block:
-                req.done_event.synchronize()
-            elif not req.done_event.query():
-                still.append(req)
-                continue
-            req.t_done = time.perf_counter()
-            self._collect(req)
-            with self._mem_cv:
-                self.slab.release(*req.slab)
-                self.desc_slab.release(*req.desc)
-                self._n_placed -= 1
-            freed = True
-            req.prog = None if self.fetch_mode != "keep" else req.prog
-            req.tasks = None
-            req.done.set()
+                ev.synchronize()
+            elif not ev.query():
+                break
+            self.finishing.pop(0)
+            t_done = time.perf_counter()
+            for req in reqs:
+                req.t_done = t_done
+                self._collect(req)
+                with self._mem_cv:
+                    self.slab.release(*req.slab)
+                    self.desc_slab.release(*req.desc)
+                    self._n_placed -= 1
+                req.prog = None if self.fetch_mode != "keep" else req.prog
+                req.tasks = None
+                req.done.set()
             with self._ready_lock:
-                self.completed.append(req)
-        self.finishing = still
+                self.completed.extend(reqs)
+            freed = True
         if freed:
             with self._mem_cv:
                 self._mem_cv.notify_all()

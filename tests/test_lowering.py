@@ -165,3 +165,38 @@ def test_solve_without_a_device_raises_instead_of_falling_back():
     m, p = ex.deterministic_workload()
     with pytest.raises(NativeKernelError):
         m.solve(params=p, log_level="off")
+
+
+def test_descriptor_pointers_address_the_right_outputs():
+    """Every packed descriptor writes its own (problem, period, regime) output
+    and reads the next period's outputs of its targets; table pointers lie
+    inside the arena. Checked on the host before any launch."""
+    from pylcm_amd import examples as ex
+    from pylcm_amd import _native as nat
+    from pylcm_amd._gs_program import GsProgram
+
+    problems = []
+    for T, n, inc in ((6, 30, 5), (4, 45, None), (6, 30, 5)):
+        m = ex.retirement_model(T, n_wealth=n, n_consumption=n + 7, income_nodes=inc)
+        inc_p = {"rho": 0.9, "sigma": 0.1, "mu": 0.0} if inc else None
+        problems.append((m, ex.retirement_params(T, crra=1.5 + 0.1 * T, income=inc_p)))
+    prog = GsProgram.of(problems, np.float32, defer=True)
+    base = 1 << 40
+    prog.base = base
+    D = prog._desc_records()
+    assert len(D) == prog.n_desc
+    for (p, t, r), di in prog.desc_of.items():
+        o = prog.outs[p][(t, r)]
+        assert D["V_out"][di] == base + o.v_off and D["pol_out"][di] == base + o.p_off
+        pt = prog.problems[p][(t, r)]
+        assert D["n_targets"][di] == len(pt.targets)
+        for k, (name, _) in enumerate(pt.targets):
+            assert D["Vn"][di][k] == base + prog.outs[p][(t + 1, name)].v_off
+        for k, (name, _) in enumerate(pt.scalars):
+            if name is not None:
+                assert D["Vs"][di][k] == base + prog.outs[p][(t + 1, name)].v_off
+        for f in ("U", "A", "B", "Af", "Bf", "Flo", "Fhi", "Fchunk", "g_pts", "w_pts"):
+            assert base <= int(D[f][di]) < base + prog.total_bytes, (f, p, t, r)
+    # relocated records differ from these by the base only
+    D0, masks = prog.relocatable_records()
+    assert np.array_equal(GsProgram.records_at(D0, masks, base), D)

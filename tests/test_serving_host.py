@@ -56,7 +56,7 @@ def test_fused_tasks_address_each_requests_own_descriptors():
         prog = GsProgram.of([(m, p)], np.float32, tile=128, defer=True)
         sl, ds = slab.alloc(prog.total_bytes), dslab.alloc(prog.n_desc * rec)
         prog.base = fake + sl[0]
-        raw = np.frombuffer(prog._desc_records(prog._descs).tobytes(), dtype=np.uint8)
+        raw = np.frombuffer(prog._desc_records().tobytes(), dtype=np.uint8)
         pool[ds[0]: ds[0] + raw.size] = raw
         placed.append((prog, sl, ds[0] // rec))
     table = np.frombuffer(pool.tobytes(), dtype=DESC_DTYPE)
@@ -108,7 +108,6 @@ def test_split_tasks_are_the_64_state_tiling():
     """With the tile chosen per launch the server splits a request's
     256-state tasks into their 64-state quarters for small launches: the same
     task set as lowering with tile 64."""
-    from types import SimpleNamespace
 
     import numpy as np
 
@@ -123,7 +122,8 @@ def test_split_tasks_are_the_64_state_tiling():
         coarse = GsProgram.of([(m, p)], np.float32, tile=256, defer=True)
         D0, _ = coarse.relocatable_records()
         base = 17  # the request's first descriptor slot in the server's table
-        req = SimpleNamespace(desc_base=base, n_w=np.asarray(D0["n_w"], dtype=np.int64))
+        n_w_of_desc = np.zeros(base + len(D0), dtype=np.int32)
+        n_w_of_desc[base:] = D0["n_w"]
         assert fine.n_steps == coarse.n_steps
         for tf, tc in zip(fine.tasks_host, coarse.tasks_host):
             want = {(int(d) + base, int(s_), int(w)) for d, s_, w in np.asarray(tf)} if len(tf) else set()
@@ -131,7 +131,7 @@ def test_split_tasks_are_the_64_state_tiling():
             if len(tc):
                 rec = np.zeros(len(tc), dtype=TASK_DTYPE)
                 rec["desc"], rec["sd"], rec["w0"] = tc[:, 0] + base, tc[:, 1], tc[:, 2]
-                out = SolveServer._split(req, rec)
+                out = SolveServer._split(n_w_of_desc, rec)
                 got = {(int(a["desc"]), int(a["sd"]), int(a["w0"])) for a in out}
             assert got == want
 
@@ -154,3 +154,81 @@ def test_grid_above_lds_capacity_is_a_clear_error():
     ok = GsProgram.of([(ex.retirement_model(3, n_wealth=2000, n_consumption=20), p)], np.float64,
                       tile=256, defer=True)
     assert max(ok.lds_by_step) <= 160 * 1024
+
+
+def test_assembled_batch_layout():
+    """Draws packed separately and assembled into one program: every task
+    addresses a descriptor of its own draw, whose pointers lie in that draw's
+    region; step s of every draw is its period T - 1 - s."""
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._batch import AssembledBatch, pack_problem
+    from pylcm_amd._gs_program import GsProgram
+
+    draws = []
+    for T, n, inc in ((5, 40, 3), (3, 60, None), (5, 35, 3)):
+        m = ex.retirement_model(T, n_wealth=n, n_consumption=n, income_nodes=inc)
+        draws.append((m, ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0} if inc else None)))
+    packed = [pack_problem(m, p, np.float32, 128) for m, p in draws]
+    ab = AssembledBatch(packed, np.float32, 128, base=1 << 40)
+    D = ab.records()
+    assert len(D) == ab.n_desc == sum(p["n_desc"] for p in packed)
+    owner = np.repeat(np.arange(len(packed)), [p["n_desc"] for p in packed])
+    for s, arr in enumerate(ab.tasks):
+        real = arr[arr["desc"] >= 0]
+        assert len(arr) % 8 == 0
+        for i, p in enumerate(packed):
+            mine = real[owner[real["desc"]] == i]
+            expect = len(p["tasks_host"][s]) if s < p["n_steps"] else 0
+            assert len(mine) == expect, (s, i)
+            if not expect:
+                continue
+            olo = ab.base + int(ab.out_offs[i])
+            tlo = ab.base + int(ab.tab_offs[i])
+            r = D[mine["desc"]]
+            assert np.all((r["V_out"] >= olo) & (r["V_out"] < olo + p["out_start"]))
+            assert np.all((r["U"] >= tlo) & (r["U"] < tlo + p["image"].size))
+            assert np.all(mine["sd"] < r["n_sd"]) and np.all(mine["w0"] < r["n_w"])
+    # a draw's records are its single-program records with outputs and tables moved
+    prog = GsProgram.of([draws[1]], np.float32, tile=128, defer=True)
+    prog.base = 0
+    single = prog._desc_records()
+    start = int(ab.desc_base[1])
+    mine = D[start: start + prog.n_desc]
+    out0 = ab.base + int(ab.out_offs[1])
+    tab0 = ab.base + int(ab.tab_offs[1]) - prog.out_start
+    assert np.array_equal(mine["V_out"], single["V_out"] + np.uint64(out0))
+    assert np.array_equal(mine["U"], single["U"] + np.uint64(tab0))
+    assert np.array_equal(mine["beta"], single["beta"]) and np.array_equal(mine["n_w"], single["n_w"])
+
+
+def test_assembled_batch_mixes_problems_with_different_pointer_fields():
+    """A deterministic problem (no income process) next to stochastic ones:
+    records relocate every pointer whatever fields each problem uses."""
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._batch import AssembledBatch, pack_problem
+    from pylcm_amd._gs_program import GsProgram
+
+    draws = [(ex.retirement_model(4, n_wealth=30, n_consumption=30), ex.retirement_params(4)),
+             (tm_model := ex.retirement_model(3, n_wealth=20, n_consumption=25, income_nodes=3),
+              ex.retirement_params(3, income={"rho": 0.8, "sigma": 0.1, "mu": 0.0}))]
+    packed = [pack_problem(m, p, np.float64, 64) for m, p in draws]
+    ab = AssembledBatch(packed, np.float64, 64, base=1 << 40)
+    D = ab.records()
+    for i, (m, p) in enumerate(draws):
+        prog = GsProgram.of([(m, p)], np.float64, tile=64, defer=True)
+        prog.base = 0
+        single = prog._desc_records()
+        mine = D[int(ab.desc_base[i]): int(ab.desc_base[i]) + prog.n_desc]
+        for f in ("V_out", "pol_out", "U", "A", "Fchunk"):
+            delta = (mine[f].astype(np.int64) - single[f].astype(np.int64))
+            lo = int(ab.out_offs[i]) if f in ("V_out", "pol_out") else int(ab.tab_offs[i]) - prog.out_start
+            assert np.all(delta == ab.base + lo), f
+        for k in range(4):
+            got, ref = mine["Vn"][:, k].astype(np.int64), single["Vn"][:, k].astype(np.int64)
+            has = ref != 0
+            assert np.all(got[has] - ref[has] == ab.base + int(ab.out_offs[i]))
+    assert tm_model.n_periods == 3
