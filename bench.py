@@ -553,11 +553,13 @@ def main():
             # at the two actions is within one float32 ulp; between one and
             # eight ulp it is graded against Q in the reference's float32
             # arithmetic (explained when that also departs from the FP64
-            # argmax or picks the GPU's index), beyond eight it is a mismatch
+            # argmax or picks the GPU's index), beyond eight it is a mismatch;
+            # on sampled periods (every state) the departures beyond one ulp
+            # are counted for the GPU and for the reference's float32 form
             Vg64 = {t: {r: np.asarray(v, dtype=np.float64) for r, v in rr.items()} for t, rr in Vg.items()}
             _, _, rep = cpu_solve.solve_c(models[s], p, grid_dtype=dt, threads=threads,
                                           next_V=Vg64, cmp_policy=Pg, reference_periods=(0, T // 2, T - 2))
-            mism += rep["mismatches"] + rep["gt_1ulp_unexplained"]
+            mism += rep["mismatches"]
             ties += rep["near_ties"]
             tie_gap = max(tie_gap, rep["max_tie_gap_ulps"])
             for k in grading:

@@ -29,6 +29,29 @@ def _align(x: int) -> int:
     return (int(x) + ALIGN - 1) // ALIGN * ALIGN
 
 
+# The descriptor's pointers fill its first PTR_WORDS 8-byte words.
+PTR_WORDS = DESC_DTYPE.fields["Bf"][1] // 8 + 1
+WORDS = DESC_DTYPE.itemsize // 8
+assert DESC_DTYPE.itemsize % 8 == 0 and all(
+    DESC_DTYPE.fields[f][1] < PTR_WORDS * 8 for f in GsProgram._PTR_FIELDS + ("Fa", "Fb", "Vn", "M", "Vs", "ms"))
+
+
+def pointer_classes(D0: np.ndarray, masks: dict, out_start: int) -> np.ndarray:
+    """(n_desc, PTR_WORDS) uint8 per pointer word of records at base 0: 0 for
+    no pointer, 1 for a pointer into the outputs (below `out_start`), 2 for a
+    pointer into the tables."""
+    n = len(D0)
+    is_ptr = np.zeros((n, PTR_WORDS), dtype=bool)
+    for f, m in masks.items():
+        w0 = DESC_DTYPE.fields[f][1] // 8
+        m2 = m.reshape(n, -1)
+        is_ptr[:, w0: w0 + m2.shape[1]] |= m2
+    words = D0.view(np.uint64).reshape(n, WORDS)[:, :PTR_WORDS]
+    cls = np.where(words < np.uint64(out_start), 1, 2).astype(np.uint8)
+    cls[~is_ptr] = 0
+    return cls
+
+
 def pack_problem(model, params, dtype, tile: int) -> dict:
     """Lower and pack one (model, params) on the host (no device code):
     everything needed to place and run it anywhere in device memory."""
@@ -37,6 +60,10 @@ def pack_problem(model, params, dtype, tile: int) -> dict:
     prog = GsProgram.of([(model, params)], dt, lowered=[low], tile=tile, defer=True)
     D0, masks = prog.relocatable_records()
     return {"image": prog._arena.image(prog.out_start), "D0": D0, "masks": masks,
+            "ptr_cls": pointer_classes(D0, masks, prog.out_start),
+            "tasks_cat": (np.concatenate(prog.tasks_host).astype(np.int32) if prog.tasks_host
+                          else np.zeros((0, 3), dtype=np.int32)),
+            "task_counts": np.array([len(t) for t in prog.tasks_host], dtype=np.int64),
             "tasks_host": prog.tasks_host, "outs": prog.outs, "n_steps": prog.n_steps,
             "lds_g": prog._lds_g, "lds_k": prog._lds_k, "wdep_by_step": prog.wdep_by_step,
             "total_bytes": prog.total_bytes, "out_start": prog.out_start, "n_desc": prog.n_desc,
@@ -141,37 +168,50 @@ class AssembledBatch:
         self.n_desc = nd
         self.desc_off = off
         off += _align(nd * DESC_DTYPE.itemsize)
-        self.tasks = []
-        self.task_off = []
-        self.lds_g, self.lds_k, self.variant = [1] * S, [0] * S, [0] * S
-        self.flops_by_step = [0.0] * S
-        for s in range(S):
-            # problems in their order (sizes interleaved, see GsProgram._pack)
-            parts, bases = [], []
+        # per-step parameters: the maximum (LDS, variant) or sum (flops) over
+        # the problems that still run at that step
+        def per_step(key, fill):
+            m = np.full((n, S), fill, dtype=np.float64)
             for i, p in enumerate(packed):
-                if s < p["n_steps"] and len(p["tasks_host"][s]):
-                    parts.append(p["tasks_host"][s])
-                    bases.append(self.desc_base[i])
-                    if p["lds_g"][s] > self.lds_g[s]:
-                        self.lds_g[s] = p["lds_g"][s]
-                    if p["lds_k"][s] > self.lds_k[s]:
-                        self.lds_k[s] = p["lds_k"][s]
-                    if p["wdep_by_step"][s] > self.variant[s]:
-                        self.variant[s] = p["wdep_by_step"][s]
-                    self.flops_by_step[s] += p["flops_by_step"][s]
-            if parts:
-                tl = np.concatenate(parts)
-                n_t = len(tl)
-                arr = np.zeros((n_t + 7) // 8 * 8, dtype=TASK_DTYPE)
-                arr["desc"][:n_t] = tl[:, 0] + np.repeat(np.asarray(bases, dtype=np.int32), [len(t) for t in parts])
-                arr["sd"][:n_t] = tl[:, 1]
-                arr["w0"][:n_t] = tl[:, 2]
-                arr["desc"][n_t:] = -1
-            else:
-                arr = np.zeros(0, dtype=TASK_DTYPE)
-            self.tasks.append(arr)
+                m[i, : p["n_steps"]] = p[key][: p["n_steps"]]
+            return m
+
+        counts = np.zeros((n, S), dtype=np.int64)
+        for i, p in enumerate(packed):
+            counts[i, : len(p["task_counts"])] = p["task_counts"]
+        live = counts > 0
+        self.lds_g = [int(v) for v in np.max(np.where(live, per_step("lds_g", 1), 1), axis=0, initial=1)]
+        self.lds_k = [int(v) for v in np.max(np.where(live, per_step("lds_k", 0), 0), axis=0, initial=0)]
+        self.variant = [int(v) for v in np.max(np.where(live, per_step("wdep_by_step", 0), 0), axis=0, initial=0)]
+        self.flops_by_step = [float(v) for v in np.sum(np.where(live, per_step("flops_by_step", 0), 0), axis=0)]
+        # every step's task list: the problems' blocks in problem order (sizes
+        # interleaved, see GsProgram._pack), descriptors offset by the
+        # problem's first descriptor, padded to a multiple of 8 with desc -1
+        per = counts.sum(axis=0)
+        padded = (per + 7) // 8 * 8
+        step_start = np.concatenate([[0], np.cumsum(padded)])
+        src_first = np.concatenate([[0], np.cumsum([len(p["tasks_cat"]) for p in packed])])[:-1]
+        src = src_first[:, None] + np.concatenate([np.zeros((n, 1), dtype=np.int64), np.cumsum(counts, axis=1)[:, :-1]],
+                                                   axis=1)
+        dst = step_start[:-1][None, :] + np.concatenate([np.zeros((1, S), dtype=np.int64),
+                                                          np.cumsum(counts, axis=0)[:-1]], axis=0)
+        cT, sT, dT = counts.T.ravel(), src.T.ravel(), dst.T.ravel()
+        total = int(step_start[-1])
+        allt = np.concatenate([p["tasks_cat"] for p in packed]) if n else np.zeros((0, 3), dtype=np.int32)
+        # gather: destination row d of block (s, i) reads source row d - (dst - src)
+        rows = np.repeat(dT, cT) + (np.arange(int(cT.sum())) - np.repeat(np.cumsum(cT) - cT, cT))
+        srows = rows - np.repeat(dT - sT, cT)
+        flat = np.zeros(total, dtype=TASK_DTYPE)
+        flat["desc"] = -1
+        dbase = np.repeat(np.broadcast_to(self.desc_base[:, None], (n, S)).T.ravel(), cT).astype(np.int32)
+        flat["desc"][rows] = allt[srows, 0] + dbase
+        flat["sd"][rows] = allt[srows, 1]
+        flat["w0"][rows] = allt[srows, 2]
+        self.tasks = [flat[step_start[s]: step_start[s + 1]] for s in range(S)]
+        self.task_off = []
+        for s in range(S):
             self.task_off.append(off)
-            off += _align(max(arr.nbytes, 1))
+            off += _align(max(self.tasks[s].nbytes, 1))
         self.total_bytes = _align(off)
         self.upload_bytes = self.total_bytes - self.tables_off
         self._graph = None
@@ -190,45 +230,58 @@ class AssembledBatch:
         if upload:
             self.upload()
 
-    def records(self) -> np.ndarray:
-        """Every problem's descriptors relocated to its place, in index order:
-        a pointer below the problem's `out_start` addresses its outputs, the
+    def records(self, into=None) -> np.ndarray:
+        """Every problem's descriptors relocated to its place, in index order
+        (written into `into`, a uint8 array of n_desc records, when given): a
+        pointer below the problem's `out_start` addresses its outputs, the
         others its tables."""
+        n = self.n_desc
+        raw = np.empty(n * DESC_DTYPE.itemsize, dtype=np.uint8) if into is None else into
+        np.concatenate([p["D0"].view(np.uint8).reshape(-1) for p in self.packed], out=raw)
         counts = [p["n_desc"] for p in self.packed]
-        D = np.concatenate([p["D0"] for p in self.packed])
         base = np.uint64(self.base)
-        out_row = np.repeat(base + self.out_offs.astype(np.uint64), counts)
-        start_row = np.repeat(np.asarray([p["out_start"] for p in self.packed], dtype=np.uint64), counts)
-        tab_row = np.repeat(base + self.tab_offs.astype(np.uint64), counts) - start_row
-        fields = set().union(*(p["masks"] for p in self.packed))
-        for f in fields:
-            # a problem without pointers in a list field has no mask for it
-            m = np.concatenate([p["masks"][f] if f in p["masks"] else
-                                np.zeros(p["D0"][f].shape, dtype=bool) for p in self.packed])
-            o = D[f]
-            if o.ndim == 1:
-                sr, orow, trow = start_row, out_row, tab_row
-            else:
-                sr, orow, trow = start_row[:, None], out_row[:, None], tab_row[:, None]
-            moved = np.where(o < sr, o + orow, o + trow)
-            D[f] = np.where(m, moved, o)
-        return D
+        lut = np.zeros((n, 3), dtype=np.uint64)
+        lut[:, 1] = np.repeat(base + self.out_offs.astype(np.uint64), counts)
+        start = np.asarray([p["out_start"] for p in self.packed], dtype=np.uint64)
+        lut[:, 2] = np.repeat(base + self.tab_offs.astype(np.uint64) - start, counts)
+        cls = np.concatenate([p["ptr_cls"] for p in self.packed])
+        W = raw.view(np.uint64).reshape(n, WORDS)
+        W[:, :PTR_WORDS] += np.take_along_axis(lut, cls.astype(np.intp), axis=1)
+        return raw.view(DESC_DTYPE)
 
     def host_image(self, into=None) -> np.ndarray:
         """Bytes [tables_off, total_bytes) of the buffer: the tables,
         descriptors and tasks (written into `into` when given)."""
         img = np.empty(self.upload_bytes, dtype=np.uint8) if into is None else into[: self.upload_bytes]
         t0 = self.tables_off
-        for i, p in enumerate(self.packed):
-            o = int(self.tab_offs[i]) - t0
-            img[o: o + p["image"].size] = p["image"]
-        D = self.records()
-        o = self.desc_off - t0
-        img[o: o + D.nbytes] = np.frombuffer(D.tobytes(), dtype=np.uint8)
+
+        def copy(lo, hi):
+            for i in range(lo, hi):
+                p = self.packed[i]
+                o = int(self.tab_offs[i]) - t0
+                img[o: o + p["image"].size] = p["image"]
+
+        n = len(self.packed)
+        k = min(8, max(1, n // 16))
+        bounds = np.linspace(0, n, k + 1).astype(int)
+        if k > 1:
+            # large copies release the GIL: the tables go in parallel
+            from concurrent.futures import ThreadPoolExecutor
+
+            with ThreadPoolExecutor(k) as ex:
+                jobs = [ex.submit(copy, int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:])]
+                o = self.desc_off - t0
+                self.records(into=img[o: o + self.n_desc * DESC_DTYPE.itemsize])
+                for j in jobs:
+                    j.result()
+        else:
+            copy(0, n)
+            o = self.desc_off - t0
+            self.records(into=img[o: o + self.n_desc * DESC_DTYPE.itemsize])
         for s, arr in enumerate(self.tasks):
             if len(arr):
                 o = self.task_off[s] - t0
-                img[o: o + arr.nbytes] = np.frombuffer(arr.tobytes(), dtype=np.uint8)
+                img[o: o + arr.nbytes] = arr.view(np.uint8).reshape(-1)
         return img
 
     def upload(self):

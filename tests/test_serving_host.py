@@ -178,6 +178,13 @@ def test_assembled_batch_layout():
     for s, arr in enumerate(ab.tasks):
         real = arr[arr["desc"] >= 0]
         assert len(arr) % 8 == 0
+        # exactly the problems' blocks in problem order, then desc -1 padding
+        blocks = [np.asarray(p["tasks_host"][s]) + [int(ab.desc_base[i]), 0, 0]
+                  for i, p in enumerate(packed) if s < p["n_steps"] and len(p["tasks_host"][s])]
+        expect_rows = np.concatenate(blocks) if blocks else np.zeros((0, 3), dtype=np.int64)
+        got_rows = np.stack([arr["desc"], arr["sd"], arr["w0"]], axis=1)
+        assert np.array_equal(got_rows[: len(expect_rows)], expect_rows), s
+        assert np.all(arr["desc"][len(expect_rows):] == -1) and len(arr) - len(expect_rows) < 8
         for i, p in enumerate(packed):
             mine = real[owner[real["desc"]] == i]
             expect = len(p["tasks_host"][s]) if s < p["n_steps"] else 0
