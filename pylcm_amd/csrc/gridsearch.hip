@@ -16,16 +16,17 @@
 // of one (descriptor, discrete state). Lanes map to states; the 256/TILE waves
 // that share a state interleave over the actions and are merged through LDS
 // at the end (ties keep the smaller flat index, so the result is the first
-// argmax whatever the interleave). Per discrete action the block stages
-//   * the contracted EV row (next-period V rows weighted by M), and
-//   * chunks of kActChunk actions' operands {A, B, U, lo} (+ hi, or affine
-//     constraint rows) in LDS.
-// The action loop index is wave-uniform, so every per-action operand is an LDS
-// broadcast read; the only per-lane LDS traffic is the two-corner gather from
-// the EV row (one ds_read2). Block-uniform conditions (continuation kind,
-// finite EV row) select a specialised action loop outside the loop. Blocks
-// are remapped so that consecutive tasks (same problem, same V rows) land on
-// the same XCD and share its L2.
+// argmax whatever the interleave). Per discrete action the block stages the
+// contracted EV row (next-period V rows weighted by M) in LDS, then walks the
+// actions in chunks of kActChunk. The action loop index is wave-uniform: in
+// the common (folded) loop every per-action operand {A', B', U, bounds} is a
+// scalar load from the global tables into SGPRs, so the only LDS traffic per
+// action is each lane's 8-byte gather of (EV_k, slope_k); the other loops stage
+// the chunk's operands {A, B, U, lo} (+ hi, or affine constraint rows) in LDS
+// and read them as broadcasts. Block-uniform conditions (continuation kind,
+// finite EV row, operand magnitudes) select a specialised action loop outside
+// the loop. Blocks are remapped so that consecutive tasks (same problem, same
+// V rows) land on the same XCD and share its L2.
 
 #include <type_traits>
 
@@ -209,6 +210,7 @@ struct LoopArgs {
   const T* Bg;
   const Pair<T>* pairs;  // folded EV row (fast variant)
   const T* pm;           // prefix max of the staged lower bounds (fast variant)
+  const T* a_uni;        // A' of the chunk when it is one value (fast variant)
   const LcmGsDesc* D;
   Grid1D<T> g;
   T w, beta, cs;
@@ -273,80 +275,121 @@ __device__ __forceinline__ void staged_loop(const LoopArgs<T>& L, Running<T>& ru
   }
 }
 
+// q where f holds, NaN elsewhere, as a select on a value every lane computed
+__device__ __forceinline__ float select_nan(bool f, float q) {
+  return __builtin_bit_cast(float, f ? __builtin_bit_cast(unsigned, q) : 0x7fc00000u);
+}
+__device__ __forceinline__ double select_nan(bool f, double q) { return f ? q : __builtin_nan(""); }
+
+// Pointers into global tables read with wave-uniform indices: loads through
+// them go to the scalar unit, so per-action operands arrive in SGPRs and cost
+// neither LDS bandwidth nor VGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using UniformPtr = const __attribute__((address_space(4))) T*;
+#else
+template <typename T>
+using UniformPtr = const T*;  // the host pass only type-checks kernel bodies
+#endif
+
 // Folded loop of the fast variant for the common case (interval feasibility,
-// linear next grid, finite EV row with at least two points). Staging folded
+// linear next grid, finite EV row with at least two points). The host folded
 // the grid into each action, so the coordinate of the next state is one fma,
-// coord = A' * w + B' with A' = A / step and B' = (B - start) / step, and
-// U' = U + beta * cs; the EV row is held as pairs (EV_k, EV_{k+1} - EV_k), so
-// Q = U' + beta * (EV_lo + t * slope). The block runs this loop only on chunks
-// whose magnitudes (|coord| <= 2^22, |EV|, |slope| <= 1e30, U' not NaN) rule
-// out overflow and NaN; the other chunks take the staged loop. LO / HI drop the
-// bound checks a chunk does not need (all lower bounds -inf / all upper
-// bounds +inf). Four actions per iteration give the scheduler independent LDS
-// reads. The update is `offer`'s first-argmax rule, with NaN as the running
-// best while no action is feasible: the first feasible action passes
-// !(q <= best) whatever its value, later ones need q > best.
-template <typename T, int SPLIT, bool LO, bool HI, bool INTERP>
-__device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& run) {
+// coord = A' * w + B' with A' = A / step and B' = (B - start) / step; the EV row
+// is held as pairs (EV_k + cs, EV_{k+1} - EV_k) with the scalar continuation
+// cs already added, so Q = U + beta * (EV_lo + t * slope), t = coord - lo, lo
+// the floor of coord clamped to [0, g_n - 2]. Every fma is written out, so the
+// recovery of a group's maximiser recomputes exactly the loop's values. The
+// per-action operands A', B', U and the bounds are read in place from the
+// global tables through UniformPtr (scalar loads); UA: A' is one value over
+// the chunk, held in a vector register. The block runs this loop only on
+// chunks whose magnitudes (|coord| <= 2^22, |EV|, |slope| <= 1e30, U not NaN)
+// rule out overflow and NaN; the other chunks take the staged loop. LO / HI
+// drop the bound checks a chunk does not need (all lower bounds -inf / all
+// upper bounds +inf). The update is `offer`'s first-argmax rule, with NaN as
+// the running best while no action is feasible: the first feasible action
+// passes !(q <= best) whatever its value, later ones need q > best.
+template <typename T, int SPLIT, bool LO, bool HI, bool UA, bool INTERP>
+__device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, UniformPtr<T> pa, UniformPtr<T> pb,
+                                              UniformPtr<T> pu, UniformPtr<T> plo, UniformPtr<T> phi,
+                                              Running<T>& run) {
   const T w = L.w;
   const T top = T(L.g_n - 2);
   const Pair<T>* __restrict__ pr = L.pairs;
   const int base = L.d * L.n_c + L.c0;
+  const T beta = L.beta;
   T best = run.idx < 0 ? T(NAN) : run.best;
   int idx = run.idx;
-  auto feas_of = [&](const Act<T>& a, int i) -> bool {
+  auto feas_of = [&](int i) -> bool {
     bool f = L.valid;
-    if constexpr (LO) f = f & (w >= a.lo);
-    if constexpr (HI) f = f & (w <= L.his[i]);
+    if constexpr (LO) f = f & (w >= plo[i]);
+    if constexpr (HI) f = f & (w <= phi[i]);
     return f;
   };
-  const T beta = L.beta;
-  // Written as plain arithmetic: explicit fmas here (__builtin_fmaf, or
-  // `fp contract(on)`) schedule 4 % slower than the default contraction. The
-  // recovery of the best quad's slot after each loop recomputes Q with this
-  // function and takes V from those same recomputed values, so V and the
-  // policy index agree whatever the contraction.
-  auto q_of = [&](const Act<T>& a) -> T {
-    if constexpr (!INTERP) return a.U;  // staged as U + beta * cs
-    const T coord = a.A * w + a.B;
+  // UA: every A' of the chunk is the one value staged in LDS, read once into a
+  // vector register, so the fma takes one scalar operand (B') and needs no copy
+  const T a_lds = UA ? L.a_uni[0] : T(0);
+  const T cs = L.cs;
+  auto q_of = [&](int i) -> T {
+    if constexpr (!INTERP) return fma_t(beta, cs, pu[i]);  // a scalar continuation
+    const T coord = fma_t(UA ? a_lds : pa[i], w, pb[i]);
     const T fl = clamp_index(T(floor(coord)), top);
     const Pair<T> p = pr[static_cast<int>(fl)];
-    return a.U + beta * (p.v + (coord - fl) * p.dv);
+    return fma_t(beta, fma_t(coord - fl, p.dv, p.v), pu[i]);
   };
   auto update = [&](T q, bool f, int c) {
     const bool take = f & !(q <= best);
     best = take ? q : best;
     idx = take ? c : idx;
   };
-  // In the four-action loops the maximiser is kept as (it, slot): the loop
-  // position of the last quad that improved and the position k in that quad,
-  // idx = it + k * SPLIT.
+  // In the grouped loops the maximiser is kept as (it, slot): the loop
+  // position of the last group that improved and the position k in that
+  // group, idx = it + k * SPLIT.
   int it = idx, slot = 0;
   if (__builtin_amdgcn_ballot_w64(L.valid) == 0) return;  // no state of this wave is real
   int i = L.grp;
   if constexpr (LO && !HI) {
-    // Actions [0, K) have every lower bound up to theirs at most the wave's
+    // Actions [0, a) have every lower bound up to theirs at most the wave's
     // smallest state, so every lane can take them: no bound checks there.
-    int a = 0, b = L.cn;
-    while (a < b) {
-      const int mid = (a + b) >> 1;
-      if (__builtin_amdgcn_readfirstlane(int(L.pm[mid] <= L.w_wave))) a = mid + 1;
-      else b = mid;
+    // pm is the chunk's prefix maximum of the lower bounds, so a counts the
+    // actions with pm <= w_wave: each lane tests a quarter of the chunk and
+    // the ballots are counted.
+    const int lane = __lane_id();
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < kActChunk / 64; ++k) {
+      const int c = lane + 64 * k;
+      a += __builtin_popcountll(__builtin_amdgcn_ballot_w64(c < L.cn && L.pm[c] <= L.w_wave));
     }
+    a = __builtin_amdgcn_readfirstlane(a);
     // Every action here is feasible and its Q is not NaN (the folded loop's
-    // gates), so a quad updates with its maximum alone and remembers only
-    // which quad improved; the position inside that quad is recovered once,
-    // after the loop, from the same Q values recomputed.
-    bool open = false;  // the best quad was taken here and its slot is not known yet
+    // gates), so a group of eight updates with its maximum alone and remembers
+    // only which group improved (the leftovers run in groups of four); the
+    // position inside the best group is recovered once, after the loops,
+    // from the same Q values recomputed.
+    bool open8 = false;
+    int it8 = it;
+    for (; i + 7 * SPLIT < a; i += 8 * SPLIT) {
+      const T q0 = q_of(i);
+      const T q1 = q_of(i + SPLIT);
+      const T q2 = q_of(i + 2 * SPLIT);
+      const T q3 = q_of(i + 3 * SPLIT);
+      const T q4 = q_of(i + 4 * SPLIT);
+      const T q5 = q_of(i + 5 * SPLIT);
+      const T q6 = q_of(i + 6 * SPLIT);
+      const T q7 = q_of(i + 7 * SPLIT);
+      const T m = fmax(fmax(fmax(q0, q1), fmax(q2, q3)), fmax(fmax(q4, q5), fmax(q6, q7)));
+      const bool take = L.valid & !(m <= best);
+      best = take ? m : best;
+      it8 = take ? base + i : it8;
+      open8 |= take;
+    }
+    bool open = false;
     for (; i + 3 * SPLIT < a; i += 4 * SPLIT) {
-      const Act<T> a0 = L.act[i];
-      const Act<T> a1 = L.act[i + SPLIT];
-      const Act<T> a2 = L.act[i + 2 * SPLIT];
-      const Act<T> a3 = L.act[i + 3 * SPLIT];
-      const T q0 = q_of(a0);
-      const T q1 = q_of(a1);
-      const T q2 = q_of(a2);
-      const T q3 = q_of(a3);
+      const T q0 = q_of(i);
+      const T q1 = q_of(i + SPLIT);
+      const T q2 = q_of(i + 2 * SPLIT);
+      const T q3 = q_of(i + 3 * SPLIT);
       const T m = fmax(fmax(q0, q1), fmax(q2, q3));
       const bool take = L.valid & !(m <= best);
       best = take ? m : best;
@@ -354,36 +397,52 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
       open |= take;
     }
     if (open) {
-      // the first position of the quad that reaches its maximum
+      // the first position of the group that reaches its maximum
       const int ql = it - base;
-      const T r0 = q_of(L.act[ql]);
-      const T r1 = q_of(L.act[ql + SPLIT]);
-      const T r2 = q_of(L.act[ql + 2 * SPLIT]);
-      const T r3 = q_of(L.act[ql + 3 * SPLIT]);
+      const T r0 = q_of(ql);
+      const T r1 = q_of(ql + SPLIT);
+      const T r2 = q_of(ql + 2 * SPLIT);
+      const T r3 = q_of(ql + 3 * SPLIT);
       const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
       slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
       best = mr;  // V from the values the slot was chosen from
+    } else if (open8) {
+      const int ql = it8 - base;
+      const T r0 = q_of(ql);
+      const T r1 = q_of(ql + SPLIT);
+      const T r2 = q_of(ql + 2 * SPLIT);
+      const T r3 = q_of(ql + 3 * SPLIT);
+      const T r4 = q_of(ql + 4 * SPLIT);
+      const T r5 = q_of(ql + 5 * SPLIT);
+      const T r6 = q_of(ql + 6 * SPLIT);
+      const T r7 = q_of(ql + 7 * SPLIT);
+      const T mr = fmax(fmax(fmax(r0, r1), fmax(r2, r3)), fmax(fmax(r4, r5), fmax(r6, r7)));
+      slot = r0 == mr ? 0 : r1 == mr ? 1 : r2 == mr ? 2 : r3 == mr ? 3 : r4 == mr ? 4 : r5 == mr ? 5 : r6 == mr ? 6 : 7;
+      it = it8;
+      best = mr;
     }
   }
+  // the same quad update with the infeasible actions entered as NaN (which
+  // the maximum skips); every lane evaluates, the clamped index keeping the
+  // read in the row, so there is no branch per action
   bool open_b = false;  // as `open` in the prefix loop
   for (; i + 3 * SPLIT < L.cn; i += 4 * SPLIT) {
-    const Act<T> a0 = L.act[i];
-    const Act<T> a1 = L.act[i + SPLIT];
-    const Act<T> a2 = L.act[i + 2 * SPLIT];
-    const Act<T> a3 = L.act[i + 3 * SPLIT];
-    const bool f0 = feas_of(a0, i);
-    const bool f1 = feas_of(a1, i + SPLIT);
-    const bool f2 = feas_of(a2, i + 2 * SPLIT);
-    const bool f3 = feas_of(a3, i + 3 * SPLIT);
-    if (__builtin_amdgcn_ballot_w64(f0 | f1 | f2 | f3) == 0) continue;  // no lane of this wave takes any of the four
-    // the same quad update with the infeasible actions as NaN (which the
-    // maximum skips); Q itself is never NaN in the folded loop
-    const T q0 = f0 ? q_of(a0) : T(NAN);
-    const T q1 = f1 ? q_of(a1) : T(NAN);
-    const T q2 = f2 ? q_of(a2) : T(NAN);
-    const T q3 = f3 ? q_of(a3) : T(NAN);
+    const bool f0 = feas_of(i);
+    const bool f1 = feas_of(i + SPLIT);
+    const bool f2 = feas_of(i + 2 * SPLIT);
+    const bool f3 = feas_of(i + 3 * SPLIT);
+    // four ballots (each a compare result already in a scalar mask)
+    if ((__builtin_amdgcn_ballot_w64(f0) | __builtin_amdgcn_ballot_w64(f1) | __builtin_amdgcn_ballot_w64(f2) |
+         __builtin_amdgcn_ballot_w64(f3)) == 0)
+      continue;
+    const T q0 = select_nan(f0, q_of(i));
+    const T q1 = select_nan(f1, q_of(i + SPLIT));
+    const T q2 = select_nan(f2, q_of(i + 2 * SPLIT));
+    const T q3 = select_nan(f3, q_of(i + 3 * SPLIT));
     const T m = fmax(fmax(q0, q1), fmax(q2, q3));
-    const bool take = (f0 | f1 | f2 | f3) & !(m <= best);
+    // m is NaN exactly when none of the four is feasible (a feasible Q is
+    // never NaN in the folded loop)
+    const bool take = (m == m) & !(m <= best);
     best = take ? m : best;
     it = take ? base + i : it;
     open_b |= take;
@@ -391,45 +450,35 @@ __device__ __forceinline__ void folded_loop(const LoopArgs<T>& L, Running<T>& ru
   if (open_b) {
     // the first feasible position of the best quad that reaches its maximum
     const int ql = it - base;
-    const Act<T> b0 = L.act[ql];
-    const Act<T> b1 = L.act[ql + SPLIT];
-    const Act<T> b2 = L.act[ql + 2 * SPLIT];
-    const Act<T> b3 = L.act[ql + 3 * SPLIT];
-    const T r0 = feas_of(b0, ql) ? q_of(b0) : T(NAN);
-    const T r1 = feas_of(b1, ql + SPLIT) ? q_of(b1) : T(NAN);
-    const T r2 = feas_of(b2, ql + 2 * SPLIT) ? q_of(b2) : T(NAN);
-    const T r3 = feas_of(b3, ql + 3 * SPLIT) ? q_of(b3) : T(NAN);
+    const T r0 = select_nan(feas_of(ql), q_of(ql));
+    const T r1 = select_nan(feas_of(ql + SPLIT), q_of(ql + SPLIT));
+    const T r2 = select_nan(feas_of(ql + 2 * SPLIT), q_of(ql + 2 * SPLIT));
+    const T r3 = select_nan(feas_of(ql + 3 * SPLIT), q_of(ql + 3 * SPLIT));
     const T mr = fmax(fmax(r0, r1), fmax(r2, r3));
     slot = r0 == mr ? 0 : (r1 == mr ? 1 : (r2 == mr ? 2 : 3));
     best = mr;  // V from the values the slot was chosen from
   }
   idx = it + slot * SPLIT;
   for (; i < L.cn; i += SPLIT) {
-    const Act<T> a = L.act[i];
-    const bool f = feas_of(a, i);
+    const bool f = feas_of(i);
     if (__builtin_amdgcn_ballot_w64(f) == 0) continue;
-    update(q_of(a), f, base + i);
+    update(q_of(i), f, base + i);
   }
   run.best = idx < 0 ? T(-INFINITY) : best;
   run.idx = idx;
 }
 
-template <typename T, int SPLIT, bool INTERP>
-__device__ __forceinline__ void folded_bounds(const LoopArgs<T>& L, bool lo, bool hi, Running<T>& run) {
+template <typename T, int SPLIT, bool UA, bool INTERP>
+__device__ __forceinline__ void folded_bounds(const LoopArgs<T>& L, bool lo, bool hi, UniformPtr<T> pa,
+                                              UniformPtr<T> pb, UniformPtr<T> pu, UniformPtr<T> plo,
+                                              UniformPtr<T> phi, Running<T>& run) {
   if (lo) {
-    if (hi) folded_loop<T, SPLIT, true, true, INTERP>(L, run);
-    else folded_loop<T, SPLIT, true, false, INTERP>(L, run);
+    if (hi) folded_loop<T, SPLIT, true, true, UA, INTERP>(L, pa, pb, pu, plo, phi, run);
+    else folded_loop<T, SPLIT, true, false, UA, INTERP>(L, pa, pb, pu, plo, phi, run);
   } else {
-    if (hi) folded_loop<T, SPLIT, false, true, INTERP>(L, run);
-    else folded_loop<T, SPLIT, false, false, INTERP>(L, run);
+    if (hi) folded_loop<T, SPLIT, false, true, UA, INTERP>(L, pa, pb, pu, plo, phi, run);
+    else folded_loop<T, SPLIT, false, false, UA, INTERP>(L, pa, pb, pu, plo, phi, run);
   }
-}
-
-template <typename T, int SPLIT>
-__device__ __forceinline__ void folded_dispatch(const LoopArgs<T>& L, bool interp, bool lo, bool hi,
-                                                Running<T>& run) {
-  if (interp) folded_bounds<T, SPLIT, true>(L, lo, hi, run);
-  else folded_bounds<T, SPLIT, false>(L, lo, hi, run);
 }
 
 // The action loop over one chunk, specialised on the kernel variant, the
@@ -521,10 +570,12 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const int b = blockIdx.x;
   const int per_xcd = n_tasks / kXcds;
   const int task_id = (b % kXcds) * per_xcd + (b / kXcds);
-  const LcmGsTask tk = tasks[task_id];
+  // descriptors and task lists are constant while the kernel runs: read them
+  // through the scalar cache whatever the kernel stores meanwhile
+  const LcmGsTask tk = UniformPtr<LcmGsTask>(tasks)[task_id];
   if (tk.desc < 0) return;  // padding task: whole block exits together
 
-  const LcmGsDesc* __restrict__ D = descs + tk.desc;
+  const UniformPtr<LcmGsDesc> D = UniformPtr<LcmGsDesc>(descs + tk.desc);
   const int tid = threadIdx.x;
   const int ls = tid % TILE;
   // the waves sharing a state interleave over actions; grp is wave-uniform
@@ -572,7 +623,9 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   L.ev = ev;
   L.pairs = pairs;
   L.pm = pm;
-  L.D = D;
+  T* a_uni = wtot + kBlock / 64;  // after the per-wave maxima
+  L.a_uni = a_uni;
+  L.D = reinterpret_cast<const LcmGsDesc*>(descs + tk.desc);
   L.w = w;
   L.valid = valid;
   L.n_constr = n_constr;
@@ -607,10 +660,10 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const int64_t wofs_b = WDEP ? int64_t(wr) * D->B_sw : 0;
 
   Running<T> run{T(-INFINITY), -1, false};
-  T qc[LCM_MAX_TASTE_D];
-  int qci[LCM_MAX_TASTE_D];
-#pragma unroll
-  for (int i = 0; i < LCM_MAX_TASTE_D; ++i) { qc[i] = -INFINITY; qci[i] = 0; }
+  // taste shocks: running logsum over the discrete actions (max tm, scaled
+  // sum ts, NaN flag), so no per-action array stays live in registers
+  T tm = T(-INFINITY), ts = T(0);
+  bool tnan = false;
 
   for (int d = 0; d < n_d; ++d) {
     T cs = T(0);
@@ -623,6 +676,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         const T v = static_cast<const T*>(D->Vs[s])[0];
         cs += zero_safe(m, v);
       }
+      const bool fold_cs = FAST && mode == kModeInterp;
       // EV[sd, d][k] = sum_r sum_j M_r[sd, d, j] * Vn_r[j, k], accumulated
       // in FP64 and rounded once to T (a few FP64 fmas per node and block)
       int local_nonfinite = 0;
@@ -637,7 +691,9 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
             if (m != T(0)) acc = fma_t(double(m), double(Vr[int64_t(j) * g_n + k]), acc);
           }
         }
-        const T ev_k = T(acc);
+        // the fast variant's folded loop reads the scalar continuation from
+        // the row (EV_k + cs, rounded once)
+        const T ev_k = fold_cs ? T(acc + double(cs)) : T(acc);
         ev[k * ES] = ev_k;
         local_nonfinite |= !is_finite(ev_k);
       }
@@ -654,7 +710,7 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         row_small = __syncthreads_or(big) == 0;
       }
     }
-    L.cs = cs;
+    L.cs = (FAST && mode == kModeInterp) ? T(0) : cs;  // folded into the row there
     L.d = d;
     L.Ug = static_cast<const T*>(D->U) + int64_t(sd) * D->U_ssd + wofs_u + int64_t(d) * n_c;
     L.Ag = interp ? static_cast<const T*>(D->A) + int64_t(sd) * D->A_ssd + wofs_a + int64_t(d) * n_c : nullptr;
@@ -680,31 +736,30 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
         if (folded) {
           __syncthreads();  // previous chunk consumed
           const int64_t fi = int64_t(sd) * D->Fi_ssd + int64_t(d) * n_c + c0;
-          int need_lo = 0, need_hi = 0, unsafe = 0;
+          int need_lo = 0, need_hi = 0, unsafe = 0, a_diff = 0;
+          const T a_first = interp ? Afg[c0] : T(0);
           T lo_me = T(-INFINITY);  // this thread's lower bound (one action per thread)
           static_assert(CH == kBlock, "one staged action per thread");
           for (int i = tid; i < cn; i += kBlock) {
-            Act<T> a;
+            const T lo_i = static_cast<const T*>(D->Flo)[fi + i];
+            const T hi_i = static_cast<const T*>(D->Fhi)[fi + i];
+            need_lo |= !(lo_i == T(-INFINITY));
+            need_hi |= !(hi_i == T(INFINITY));
+            lo_me = lo_i == lo_i ? lo_i : T(INFINITY);  // a NaN bound is never met
+            // the loop reads its operands in place (A' = A / step and
+            // B' = (B - start) / step folded by the host, each rounded once, and
+            // U); here only the gates: |coord| over the block's states (w_pts
+            // increase), Q of a scalar continuation not NaN, and whether A' is
+            // one value over the chunk
+            const T U = L.Ug[c0 + i];
             if (interp) {
-              // A' = A / step and B' = (B - start) / step come folded by the
-              // host (each rounded once); U' = U + beta * cs is one fma
-              a.A = Afg[c0 + i];
-              a.B = Bfg[c0 + i];
-              a.U = fma_t(L.beta, cs, L.Ug[c0 + i]);
-              // |coord| over the block's states (w_pts increase) and U' not NaN
-              unsafe |= !(fabs(a.A) * w_abs + fabs(a.B) <= T(4194304)) | (a.U != a.U);
+              const T A = Afg[c0 + i], B = Bfg[c0 + i];
+              unsafe |= !(fabs(A) * w_abs + fabs(B) <= T(4194304)) | (U != U);
+              a_diff |= !(A == a_first);
             } else {
-              a.A = a.B = T(0);
-              a.U = fma_t(L.beta, cs, L.Ug[c0 + i]);  // Q of a scalar continuation
-              unsafe |= a.U != a.U;
+              const T q = fma_t(L.beta, cs, U);
+              unsafe |= q != q;
             }
-            a.lo = static_cast<const T*>(D->Flo)[fi + i];
-            const T hi = static_cast<const T*>(D->Fhi)[fi + i];
-            act[i] = a;
-            his[i] = hi;
-            need_lo |= !(a.lo == T(-INFINITY));
-            need_hi |= !(hi == T(INFINITY));
-            lo_me = a.lo == a.lo ? a.lo : T(INFINITY);  // a NaN bound is never met
           }
           // inclusive prefix max of the lower bounds: within the wave by
           // shuffles, then across waves through LDS
@@ -715,12 +770,25 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
             if (lane >= off) lo_me = fmax(lo_me, y);
           }
           if (lane == 63) wtot[tid >> 6] = lo_me;
-          const bool lo = __syncthreads_or(need_lo) != 0;
+          const bool lo = __builtin_amdgcn_readfirstlane(__syncthreads_or(need_lo)) != 0;
           for (int k = 0; k < (tid >> 6); ++k) lo_me = fmax(lo_me, wtot[k]);
           if (tid < cn) pm[tid] = lo_me;
-          const bool hi = __syncthreads_or(need_hi) != 0;
-          if (__syncthreads_or(unsafe) == 0) {
-            folded_dispatch<T, SPLIT>(L, interp, lo, hi, run);
+          const bool hi = __builtin_amdgcn_readfirstlane(__syncthreads_or(need_hi)) != 0;
+          if (__builtin_amdgcn_readfirstlane(__syncthreads_or(unsafe)) == 0) {
+            const UniformPtr<T> pa = UniformPtr<T>(interp ? Afg + c0 : nullptr);
+            const UniformPtr<T> pb = UniformPtr<T>(interp ? Bfg + c0 : nullptr);
+            const UniformPtr<T> pu = UniformPtr<T>(L.Ug + c0);
+            const UniformPtr<T> plo = UniformPtr<T>(static_cast<const T*>(D->Flo) + fi);
+            const UniformPtr<T> phi = UniformPtr<T>(static_cast<const T*>(D->Fhi) + fi);
+            if (interp) {
+              if (tid == 0) a_uni[0] = a_first;
+              if (__builtin_amdgcn_readfirstlane(__syncthreads_or(a_diff)) == 0)
+                folded_bounds<T, SPLIT, true, true>(L, lo, hi, pa, pb, pu, plo, phi, run);
+              else
+                folded_bounds<T, SPLIT, false, true>(L, lo, hi, pa, pb, pu, plo, phi, run);
+            } else {
+              folded_bounds<T, SPLIT, false, false>(L, lo, hi, pa, pb, pu, plo, phi, run);
+            }
             continue;
           }
           // otherwise the staged chunk below replaces the folded one
@@ -759,9 +827,18 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
       block_merge<T, TILE>(run, red_v, red_i, red_f, tid, ls, grp);
       const T qd = run.nan ? T(NAN) : (run.idx >= 0 ? run.best : T(-INFINITY));
       const int id = (run.idx >= 0 && !run.nan) ? run.idx : 0;
-#pragma unroll
-      for (int i = 0; i < LCM_MAX_TASTE_D; ++i) {
-        if (i == d) { qc[i] = qd; qci[i] = id; }
+      if (grp == 0 && valid) {
+        D->pol_out[(int64_t(sd) * n_w + wi) * n_d + d] = id;
+        // logsum_and_softmax (logsum.py:32-64), m + scale * log(sum exp((q - m) / scale)),
+        // accumulated one discrete action at a time: the sum is rescaled
+        // whenever the maximum rises
+        const T lam = T(D->taste_scale);
+        tnan |= qd != qd;
+        if (qd == qd) {
+          const T m1 = fmax(tm, qd);
+          if (m1 != T(-INFINITY)) ts = ts * exp((tm - m1) / lam) + exp((qd - m1) / lam);
+          tm = m1;
+        }
       }
       run = Running<T>{T(-INFINITY), -1, false};
     }
@@ -769,30 +846,11 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
 
   if (taste) {
     if (grp == 0 && valid) {
-      // logsum_and_softmax (logsum.py:32-64): m + scale * log(sum exp((q - m)/scale))
       const T lam = T(D->taste_scale);
-      T m = -INFINITY;
-      bool has_nan = false;
-#pragma unroll
-      for (int i = 0; i < LCM_MAX_TASTE_D; ++i) {
-        if (i < n_d) {
-          has_nan |= qc[i] != qc[i];
-          m = fmax(m, qc[i]);
-        }
-      }
-      const T fm = (m == -INFINITY) ? T(0) : m;
-      T s = T(0);
-#pragma unroll
-      for (int i = 0; i < LCM_MAX_TASTE_D; ++i) {
-        if (i < n_d) s += exp((qc[i] - fm) / lam);
-      }
-      T V = fm + lam * log(s);
-      if (has_nan) V = T(NAN);
-      const int64_t o = int64_t(sd) * n_w + wi;
-      static_cast<T*>(D->V_out)[o] = V;
-      for (int i = 0; i < LCM_MAX_TASTE_D; ++i) {
-        if (i < n_d) D->pol_out[o * n_d + i] = qci[i];
-      }
+      const T fm = (tm == T(-INFINITY)) ? T(0) : tm;
+      T V = fm + lam * log(ts);
+      if (tnan) V = T(NAN);
+      static_cast<T*>(D->V_out)[int64_t(sd) * n_w + wi] = V;
     }
     return;
   }
