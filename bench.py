@@ -369,6 +369,35 @@ def serving_leg(args):
     }
 
 
+def tiles_leg(draws, models, lowered, ctx):
+    """The autotuned tile against every tile and the untuned fallback, for one
+    draw of each grid size solved alone (the plain path's launches) and for the
+    whole batch: mean per-launch time of the search steps (HIP events)."""
+    from pylcm_amd import autotune
+    from pylcm_amd._gs_program import GsProgram
+
+    def per_launch(probs, low, tile):
+        prog = GsProgram.of(probs, np.float32, lowered=low, tile=tile, ctx=ctx)
+        prog.run(use_graph=False)
+        best = min(float(np.mean([m for s, m in prog.time_launches() if s > 0])) for _ in range(3))
+        return prog, best
+
+    out = {"single": {}}
+    for size in sorted({s for s, _ in draws}):
+        i = next(k for k, (s, _) in enumerate(draws) if s == size)
+        probs, low = [(models[size], draws[i][1])], [lowered[i]]
+        tuned, _ = per_launch(probs, low, None)
+        units, n_actions = tuned.launch_shape()
+        times = {t: per_launch(probs, low, t)[1] for t in autotune.TILES}
+        fb = autotune.fallback_tile(units)
+        out["single"][str(size)] = {
+            "units": units, "tuned_tile": tuned.tile, "fallback_tile": fb,
+            "ms_per_launch": {str(t): round(v, 4) for t, v in times.items()},
+            "gain_vs_fallback": round(times[fb] / times[tuned.tile], 3)}
+        log(f"tiles, one draw of grid {size}: {out['single'][str(size)]}")
+    return out
+
+
 def fresh_leg(args, models, T, tile):
     """Fresh draws: every timed batch holds NEW parameter draws of the batched
     workload, lowered and packed on the host by worker processes and placed
@@ -574,6 +603,7 @@ def main():
                   **grading, "checked_draws": len(order)}
         log(f"cpu baseline: {cpu['value']:.3f} solves/s on {threads} threads; parity {parity}")
 
+    tiles = tiles_leg(draws, models, lowered, prog.ctx)
     fresh = fresh_leg(args, models, T, prog.tile) if args.fresh_batches > 0 else None
     dcegm = None if args.no_dcegm else dcegm_leg(args)
     simulation = None if args.no_sim else simulation_leg(args)
@@ -600,6 +630,8 @@ def main():
                         "period replayed as a HIP graph",
             "draws": args.draws, "periods": T, "tile": prog.tile,
         },
+        "tiles": dict(tiles, batch_tuned_tile=prog.tile, batch_shape=dict(zip(("units", "n_actions"),
+                                                                             prog.launch_shape()))),
         "plain_gpu": {
             "solves_per_s": round(plain_sps, 3),
             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 4),

@@ -22,6 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from pylcm_amd import _native as nat
+from pylcm_amd import autotune
 from pylcm_amd._build import PeriodLowering, resolve_grids, table_digest
 from pylcm_amd.exceptions import UnsupportedModelError
 
@@ -165,15 +166,6 @@ def folded_next_state(A, B, g_start, g_stop, g_n, dt):
             ((B64 - np.float64(start)) / np.float64(step)).astype(dt))
 
 
-def default_tile(n_states_total: int) -> int:
-    """States per workgroup: small workloads split actions over more waves."""
-    if n_states_total <= 64 * 1024:
-        return 64
-    if n_states_total <= 512 * 1024:
-        return 128
-    return 256
-
-
 def lower_problem(model, params, dtype):
     """(tables by (period, regime), resolved grids) for one params set."""
     rg = resolve_grids(model.layouts, params, model.fixed_params, dtype)
@@ -241,12 +233,7 @@ class GsProgram:
         self._rg = [rg for _, rg in low]
         self.lower_seconds = time.perf_counter() - t0
         self.n_steps = max(m.n_periods for m in self.models)
-        n_states = sum(
-            m.layouts[r].n_sd * m.layouts[r].n_w
-            for m, tabs in zip(self.models, self.problems) for (t, r) in tabs
-            if t == m.n_periods - 2
-        ) or 1
-        self.tile = tile or default_tile(n_states)
+        self.tile = tile or self.tuned_tile()
         self._graph = None
         t0 = time.perf_counter()
         self._pack()
@@ -255,6 +242,24 @@ class GsProgram:
     @property
     def model(self):
         return self.models[0]
+
+    def launch_shape(self) -> tuple[int, int]:
+        """(work units, actions per state) of a search step: 256-state tasks of
+        the regimes every problem solves in its next-to-last period, and the
+        largest action count."""
+        units, n_actions = 0, 1
+        for m, tabs in zip(self.models, self.problems):
+            for (t, r) in tabs:
+                if t == max(m.n_periods - 2, 0):
+                    lay = m.layouts[r]
+                    units += lay.n_sd * -(-lay.n_w // 256)
+                    n_actions = max(n_actions, lay.n_d * lay.n_c)
+        return max(units, 1), n_actions
+
+    def tuned_tile(self) -> int:
+        """The autotuned state tile of this program's launches."""
+        units, n_actions = self.launch_shape()
+        return autotune.choose("f64" if self.dt == np.float64 else "f32", units, n_actions)
 
     def step_of(self, p: int, t: int) -> int:
         return self.models[p].n_periods - 1 - t

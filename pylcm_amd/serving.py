@@ -29,6 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from pylcm_amd import _native as nat
+from pylcm_amd import autotune
 from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_problem, step_lds_bytes
 
 ALIGN = 256
@@ -129,16 +130,18 @@ class _Cohort:
     once (member p runs steps 0 .. n_steps_p - 1), so a fused launch costs one
     array per cohort rather than one per request."""
 
-    def __init__(self, members: list):
+    def __init__(self, members: list, n_act_of_desc):
         self.members = members
         self.n_steps = max(r.prog.n_steps for r in members)
-        self.steps, self.lds, self.variant = [], [], []
+        self.steps, self.lds, self.variant, self.n_actions = [], [], [], []
         self._ending = {}
         for r in members:
             self._ending.setdefault(r.prog.n_steps - 1, []).append(r)
         for s in range(self.n_steps):
             parts = [r.tasks[s] for r in members if s < r.prog.n_steps and len(r.tasks[s])]
             self.steps.append(np.concatenate(parts) if parts else np.zeros(0, dtype=TASK_DTYPE))
+            arr = self.steps[-1]
+            self.n_actions.append(int(n_act_of_desc[arr["desc"]].max()) if len(arr) else 1)
             self.lds.append(max([r.prog.lds_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
             self.variant.append(max([r.prog.wdep_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
         self.s = 0
@@ -152,18 +155,21 @@ class _Cohort:
 
 
 class SolveServer:
-    FINE_TILE, COARSE_TILE = 64, 256
+    COARSE_TILE = 256  # the tile requests are lowered with
 
     """Continuous batching of grid-search solves on one GPU.
 
     `budget_bytes` caps the device memory the server uses (default: the given
     fraction of the free HBM at start). `tile` is the state tile of the fused
-    launch (one shape per launch). `tile=0` picks it per launch: a launch with
-    fewer than `fine_below` 256-state tasks splits each into its 64-state
-    tasks (four waves share each task's actions, so a small launch has four
-    times the workgroups); a larger one runs the 256-state tasks (a quarter of
-    the per-task staging). The tile does not change results: the maximum and
-    its first maximiser are merged the same way whatever the split. `max_inflight` caps admitted requests.
+    launch (one shape per launch). `tile=0` picks it per launch with the
+    autotuner (`pylcm_amd.autotune.choose` over `tile_table`, default the
+    table measured on this GPU model) from the launch's 256-state tasks and
+    its largest action count: requests are lowered into 256-state tasks, and a
+    launch that runs a finer tile splits each into its 128- or 64-state parts
+    (two or four waves then share each task's actions, so a small launch has
+    more workgroups). The tile does not change results: the maximum and its
+    first maximiser are merged the same way whatever the split. `max_inflight`
+    caps admitted requests.
     `lower_procs` > 0 lowers requests in that many spawned worker processes
     (models are sent with cloudpickle); as with any spawned pool, the main
     script must guard its entry point with `if __name__ == "__main__":`.
@@ -172,12 +178,12 @@ class SolveServer:
     def __init__(self, *, dtype=np.float32, tile: int = 0, budget_bytes: int | None = None,
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1,
-                 lower_procs: int = 0, fine_below: int = 256):
+                 lower_procs: int = 0, tile_table: dict | None = None):
         self.dt = np.dtype(dtype)
         self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         self.auto_tile = int(tile) == 0
         self.tile = self.COARSE_TILE if self.auto_tile else int(tile)  # the lowering tile
-        self.fine_below = int(fine_below)
+        self.tile_table = tile_table  # None: the committed table
         self.tiles_used = {}  # tile -> fused launches
         self.fetch_mode = fetch
         self.ctx = nat.Context()        # kernels and task uploads
@@ -193,7 +199,8 @@ class SolveServer:
         self.desc_buf = nat.DeviceBuffer(self.desc_slab.size)
         self.max_tasks = int(max_tasks)
         self.max_inflight = int(max_inflight)
-        self.n_w_of_desc = np.zeros(int(max_desc), dtype=np.int32)  # for splitting tasks into fine tiles
+        self.n_w_of_desc = np.zeros(int(max_desc), dtype=np.int32)  # for splitting tasks into finer tiles
+        self.n_act_of_desc = np.ones(int(max_desc), dtype=np.int64)  # actions per state, for the tile choice
         # at most two fused launches queued on the stream: while one runs, the
         # next one gathers every step that becomes ready, so launches stay large
         self.n_ring = 2
@@ -320,6 +327,7 @@ class SolveServer:
         req.tasks = steps
         req.desc_base = desc_base
         self.n_w_of_desc[desc_base: desc_base + prog.n_desc] = D0["n_w"]
+        self.n_act_of_desc[desc_base: desc_base + prog.n_desc] = D0["n_d"].astype(np.int64) * D0["n_c"]
         req.slab, req.desc = sl, ds
         req.t_admit = time.perf_counter()
         with self._ready_lock:
@@ -330,7 +338,7 @@ class SolveServer:
         with self._ready_lock:
             ready, self._ready = self._ready, []
         if ready:
-            self.inflight.append(_Cohort(ready))
+            self.inflight.append(_Cohort(ready, self.n_act_of_desc))
 
     def _inflight_requests(self) -> int:
         return sum(c.remaining for c in self.inflight)
@@ -342,7 +350,7 @@ class SolveServer:
         if self.ring_used[k]:
             self.ring_events[k].synchronize()  # the copy from this host slot has finished
         rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
-        picks, n_coarse, lds, variant = [], 0, 0, 0
+        picks, n_coarse, lds, variant, n_act = [], 0, 0, 0, 1
         for c in self.inflight:
             arr = c.steps[c.s]
             if n_coarse + len(arr) + 8 > self.max_tasks:
@@ -351,6 +359,7 @@ class SolveServer:
             n_coarse += len(arr)
             lds = max(lds, c.lds[c.s])
             variant = max(variant, c.variant[c.s])
+            n_act = max(n_act, c.n_actions[c.s])
         if not picks:
             return False
         if n_coarse == 0:  # nothing to run this step (no regime of these cohorts is active)
@@ -363,11 +372,16 @@ class SolveServer:
                 c.s += 1
             self.inflight = [c for c in self.inflight if c.s < c.n_steps]
             return True
-        fine = self.auto_tile and n_coarse < self.fine_below and 4 * n_coarse + 8 <= self.max_tasks
-        tile = self.FINE_TILE if fine else self.tile
+        tile = self.tile
+        if self.auto_tile:
+            # the autotuned tile of this launch's shape (its 256-state tasks
+            # and the most actions per state), when its split tasks fit
+            tile = autotune.choose("f64" if self.dt == np.float64 else "f32", n_coarse, n_act, self.tile_table)
+            if (self.COARSE_TILE // tile) * n_coarse + 8 > self.max_tasks:
+                tile = self.COARSE_TILE
         n = 0
         for c in picks:
-            t = self._split(self.n_w_of_desc, c.steps[c.s]) if fine else c.steps[c.s]
+            t = self._split(self.n_w_of_desc, c.steps[c.s], tile) if tile < self.COARSE_TILE else c.steps[c.s]
             rec[n: n + len(t)] = t
             n += len(t)
         self.tiles_used[tile] = self.tiles_used.get(tile, 0) + 1
@@ -397,12 +411,12 @@ class SolveServer:
         return True
 
     @classmethod
-    def _split(cls, n_w_of_desc, t):
-        """The 64-state tasks of 256-state tasks `t`: each tile's four
-        quarters that start below its descriptor's n_w."""
-        k = cls.COARSE_TILE // cls.FINE_TILE
+    def _split(cls, n_w_of_desc, t, tile: int = 64):
+        """The `tile`-state tasks of 256-state tasks `t`: the parts of each
+        that start below its descriptor's n_w."""
+        k = cls.COARSE_TILE // tile
         r = np.repeat(t, k)
-        r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * cls.FINE_TILE, len(t))
+        r["w0"] += np.tile(np.arange(k, dtype=r["w0"].dtype) * tile, len(t))
         return r[r["w0"] < n_w_of_desc[r["desc"]]]
 
     def _retire(self, block: bool = False):

@@ -1,5 +1,7 @@
 """Host-side pieces of the solve server (CPU): the device-slab allocator."""
 
+import pytest
+
 from pylcm_amd.serving import _Slab
 
 
@@ -104,10 +106,11 @@ def test_worker_process_lowering_matches_in_process():
         assert got["outs"][0].keys() == prog.outs[0].keys()
 
 
-def test_split_tasks_are_the_64_state_tiling():
+@pytest.mark.parametrize("tile", [64, 128])
+def test_split_tasks_are_the_finer_tiling(tile):
     """With the tile chosen per launch the server splits a request's
-    256-state tasks into their 64-state quarters for small launches: the same
-    task set as lowering with tile 64."""
+    256-state tasks into their 64- or 128-state parts for launches that run a
+    finer tile: the same task set as lowering with that tile."""
 
     import numpy as np
 
@@ -118,7 +121,7 @@ def test_split_tasks_are_the_64_state_tiling():
     for T, g, nodes in ((5, 700, 3), (4, 257, None), (6, 64, 5), (3, 1000, None)):
         m = ex.retirement_model(T, n_wealth=g, n_consumption=50, income_nodes=nodes)
         p = ex.retirement_params(T, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0} if nodes else None)
-        fine = GsProgram.of([(m, p)], np.float32, tile=64, defer=True)
+        fine = GsProgram.of([(m, p)], np.float32, tile=tile, defer=True)
         coarse = GsProgram.of([(m, p)], np.float32, tile=256, defer=True)
         D0, _ = coarse.relocatable_records()
         base = 17  # the request's first descriptor slot in the server's table
@@ -131,7 +134,7 @@ def test_split_tasks_are_the_64_state_tiling():
             if len(tc):
                 rec = np.zeros(len(tc), dtype=TASK_DTYPE)
                 rec["desc"], rec["sd"], rec["w0"] = tc[:, 0] + base, tc[:, 1], tc[:, 2]
-                out = SolveServer._split(n_w_of_desc, rec)
+                out = SolveServer._split(n_w_of_desc, rec, tile)
                 got = {(int(a["desc"]), int(a["sd"]), int(a["w0"])) for a in out}
             assert got == want
 
