@@ -61,6 +61,42 @@ def committed_traffic(key: str):
         return None
 
 
+def _host_cores() -> int:
+    return len(os.sched_getaffinity(0))
+
+
+def _pool(procs):
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+
+    return ProcessPoolExecutor(max_workers=procs, mp_context=multiprocessing.get_context("spawn"))
+
+
+def _cpu_dcegm_solve(job):
+    """One FP64 oracle DC-EGM solve (CPU baseline worker; no device code)."""
+    from oracle import dcegm as od
+    from pylcm_amd import examples as ex
+
+    T, beta, crra, sigma = job
+    m = ex.dcegm_retirement_model(T)
+    p = ex.dcegm_params(T, discount_factor=beta, crra=crra, sigma=sigma)
+    t0 = time.perf_counter()
+    od.solve(m, p, grid_dtype=np.float32)
+    return time.perf_counter() - t0
+
+
+def _cpu_simulate(job):
+    """The FP64 oracle simulation of a slice of agents (CPU baseline worker)."""
+    from oracle import simulate as osim
+    from pylcm_amd import examples as ex
+
+    V, sub, seed = job
+    m, p = ex.stochastic_workload()
+    t0 = time.perf_counter()
+    out = osim.simulate(m, p, V, sub, seed, grid_dtype=np.float32)
+    return time.perf_counter() - t0, out["action"]
+
+
 def dcegm_leg(args):
     """DC-EGM retirement model with FUES: T=50, 2000 exponentially spaced
     savings nodes on [0, 500], 2000 wealth nodes, taste shocks 0.05, log-normal
@@ -145,9 +181,23 @@ def dcegm_leg(args):
                         "vs_single": round((args.dcegm_batch / ms_batch) * ms_solve, 2),
                         "host_build_s": round(bbuild, 2), "device_MiB": round(batch.total_bytes / 2**20, 1),
                         "draw0_equals_single_solve": bool(same)}
+        out["batch"]["columns_per_s"] = round(out["batch"]["columns_per_launch"] * (T - 1) / (ms_batch * 1e-3), 1)
         log(f"dcegm batch: {args.dcegm_batch} draws in {ms_batch:.2f} ms -> {out['batch']['solves_per_s']:.1f} solves/s "
             f"({out['batch']['vs_single']}x single); draw 0 equals the single solve: {same}")
         del batch
+    # the batch's dominant kernel: its VALU issue fraction from the committed
+    # counter pass of this workload (latency-bound bracket searches)
+    frac = None
+    prof = ROOT / "profiles" / "egm_kernels_dcegm_T50_2000x2000_sq_counters.md"
+    if prof.exists():
+        for line in prof.read_text().splitlines():
+            if line.startswith("- valu_issue_frac_euler_batch:"):
+                frac = float(line.split(":", 1)[1].strip().strip("`"))
+    out["roofline"] = {"kernel": "egm_euler_kernel<256> (the batch's continuation reads + Euler inversion)",
+                       "bound": "latency (dependent loads of bracket searches in the child carry rows)",
+                       "columns_per_s_single": out["columns_per_s"],
+                       "columns_per_s_batch": out.get("batch", {}).get("columns_per_s"),
+                       "valu_issue_frac": frac, "source": str(prof.relative_to(ROOT)) if frac else None}
     if not args.no_cpu:
         from oracle import dcegm as od
         from tests.parity import rel_err
@@ -171,13 +221,22 @@ def dcegm_leg(args):
                         same_kept &= int(cols["counts"][c, 0]) == oc["n_kept"]
                         for k in ("grid", "policy", "value"):
                             worst_rows = max(worst_rows, rel_err(cols[k][c], oc[k]))
-        per_period = cpu_s / (Ts - 1)
         out["parity_T3_full_grids"] = {"max_rel_V_err_vs_fp64": worst_v, "max_rel_carry_row_err": worst_rows,
                                        "same_kept_counts": bool(same_kept)}
-        out["cpu_baseline"] = {"value": round(1.0 / (per_period * (T - 1)), 6), "unit": "model-solves/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"NumPy FP64 restatement (oracle/dcegm.py) of the same grids with T={Ts} "
-                                         f"({Ts - 1} periods, {cpu_s:.1f}s), scaled to {T - 1} periods"}
+        # all host cores: one oracle solve per core, each its own parameter draw
+        cores = _host_cores()
+        rng = np.random.default_rng(5)
+        jobs = [(Ts, float(rng.uniform(0.93, 0.98)), float(rng.uniform(1.5, 3.0)), float(rng.uniform(0.1, 0.2)))
+                for _ in range(cores)]
+        with _pool(cores) as pool:
+            t0 = time.perf_counter()
+            list(pool.map(_cpu_dcegm_solve, jobs))
+            wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cores / (wall * (T - 1) / (Ts - 1)), 6), "unit": "model-solves/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"NumPy FP64 restatement (oracle/dcegm.py) of the same grids with T={Ts}, "
+                                         f"{cores} parameter draws in {cores} processes ({Ts - 1} periods each, "
+                                         f"{wall:.1f}s wall), scaled to {T - 1} periods; one solve alone took {cpu_s:.1f}s"}
         log(f"dcegm parity (T={Ts}): V {worst_v:.2e}, carry rows {worst_rows:.2e}, kept counts equal {same_kept}; "
             f"cpu {cpu_s:.1f}s")
     return out
@@ -227,7 +286,21 @@ def simulation_leg(args):
         tot_ms.append(evs[0][0].elapsed_ms(evs[1][-1][1]))
     res = sim.collect(ic)
     raw = res.raw
-    tot = float(np.median(tot_ms))
+    eager_tot = float(np.median(tot_ms))
+    # the same simulation replayed as one HIP graph (kernel nodes only)
+    sim.launch_graph()
+    prog.ctx.sync()
+    same_graph = all(np.array_equal(sim.collect(ic).raw[k], raw[k], equal_nan=True) for k in raw)
+    g_ms = []
+    for _ in range(reps):
+        sim._reset()
+        e0, e1 = nat.Event(), nat.Event()
+        e0.record(prog.ctx)
+        sim.capture_graph().launch(prog.ctx)
+        e1.record(prog.ctx)
+        prog.ctx.sync()
+        g_ms.append(e0.elapsed_ms(e1))
+    tot = float(np.median(g_ms))
     agent_periods = int((raw["action"] >= 0).sum())
     # algorithmic work: n_actions x 10 flops per simulated agent-period
     names = list(m.regime_names)
@@ -252,6 +325,8 @@ def simulation_leg(args):
                     "nodes), V resident from the GPU solve, Philox4x32-10 draws",
         "agents": n, "periods": T,
         "ms_per_simulation": round(tot, 3),
+        "ms_per_simulation_eager": round(eager_tot, 3),
+        "graph_equals_eager": bool(same_graph),
         "agent_periods_per_s": round(agent_periods / (tot * 1e-3), 1),
         "host_prepare_s": round(prep_s, 3),
         "roofline": {"kernel": "sim_kernel<float> (+ per-period partition kernels)", "bound": "valu",
@@ -261,25 +336,40 @@ def simulation_leg(args):
         "ev_kernel": {"ms": round(ev, 4), "bytes": ev_bytes, "achieved_GBs": round(ev_bytes / (ev * 1e-3) / 1e9, 2),
                       "peak_GBs": 8000, "bound": "hbm"},
     }
-    log(f"simulation: {n} agents x {T} periods in {tot:.2f} ms ({out['agent_periods_per_s']:.3g} agent-periods/s); "
+    log(f"simulation: {n} agents x {T} periods in {tot:.2f} ms as a graph ({eager_tot:.2f} eager; "
+        f"{out['agent_periods_per_s']:.3g} agent-periods/s); "
         f"sim kernels {sim_ms:.2f} ms, {out['roofline']['achieved']} TFLOP/s; EV {ev:.3f} ms")
     if not args.no_cpu:
-        from oracle import simulate as osim
-
         k = min(args.sim_cpu_agents, n)
         prog.fetch()
         Vg = {t: {r: v for r, (v, _) in rr.items()} for t, rr in prog.results(0).items()}
-        sub = {key: val[:k] for key, val in ic_user.items()}
-        t0 = time.perf_counter()
-        exp = osim.simulate(m, p, Vg, sub, seed, grid_dtype=dt)
-        cpu_s = time.perf_counter() - t0
-        cpu_ap = int((exp["action"] >= 0).sum())
-        same = simulated[:, :k] == (exp["action"] >= 0)
-        both = simulated[:, :k] & (exp["action"] >= 0)
-        act_mism = int((raw["action"][:, :k][both] != exp["action"][both]).sum())
-        out["cpu_baseline"] = {"value": round(cpu_ap / cpu_s, 1), "unit": "agent-periods/s", "cores": 1,
+        # all host cores: each process simulates its own slice of k agents
+        # (the oracle numbers a slice's agents from 0, so slices after the
+        # first draw other paths than the device's agents, with the same work)
+        cores = _host_cores()
+        kk = int(min(k * cores, n))
+        bounds = np.linspace(0, kk, cores + 1).astype(int)
+        jobs = []
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            sub = {key: val[a:b] for key, val in ic_user.items()}
+            jobs.append((Vg, sub, seed))
+        with _pool(cores) as pool:
+            t0 = time.perf_counter()
+            parts = list(pool.map(_cpu_simulate, jobs))
+            cpu_s = time.perf_counter() - t0
+        # each slice's agents are numbered from 0 in its own run: compare the
+        # first slice (the same agent numbers as the device's first agents)
+        exp_action = parts[0][1]
+        k0 = int(bounds[1])
+        cpu_ap = int(sum((a >= 0).sum() for _, a in parts))
+        same = simulated[:, :k0] == (exp_action >= 0)
+        both = simulated[:, :k0] & (exp_action >= 0)
+        act_mism = int((raw["action"][:, :k0][both] != exp_action[both]).sum())
+        k = k0
+        out["cpu_baseline"] = {"value": round(cpu_ap / cpu_s, 1), "unit": "agent-periods/s", "cores": cores,
                                "kind": "port",
-                               "sample": f"first {k} agents, NumPy FP64 restatement oracle/simulate.py, {cpu_s:.1f}s"}
+                               "sample": f"{kk} agents in {cores} processes ({k0} each), NumPy FP64 restatement "
+                                         f"oracle/simulate.py, {cpu_s:.1f}s wall"}
         out["parity_sample"] = {"agents": k, "same_simulated_mask": bool(same.all()),
                                 "action_mismatches": act_mism, "agent_periods": int(both.sum())}
         log(f"simulation cpu baseline: {out['cpu_baseline']['value']} agent-periods/s; parity {out['parity_sample']}")

@@ -668,20 +668,41 @@ class SimProgram:
             ("cur", "w", w0.astype(dt)), ("cur", "z", z0.astype(dt)), ("cur", "sdd", sdd0),
             ("cur", "regime", reg0), ("cur", "start", ic["_start"]), ("nxt", "start", ic["_start"]))]
         self._launched = False
+        self._graph = None
 
     def launch(self, events=None):
         """Enqueue the EV rows and the T period launches on the solve
         program's stream (no host sync). With `events` = (ev_pair, [pair per
         period]) of `nat.Event`, records them around the EV launch and each
         period."""
+        self._reset()
+        self._enqueue(events)
+
+    def _reset(self):
+        """A repeated run starts from the same initial panel (synchronous
+        uploads, never part of a captured graph)."""
+        if self._launched:
+            base = self._st["base"]
+            for o, arr in self._init_state:
+                nat.h2d(self.prog.ctx, base + o, arr)
+        self._launched = True
+
+    def capture_graph(self):
+        """The EV launch and the T period launches captured once as a HIP graph
+        (kernel nodes only: the scratch counters are cleared by a kernel)."""
+        if self._graph is None:
+            self._graph = nat.Graph.capture(self.prog.ctx, self._enqueue)
+        return self._graph
+
+    def launch_graph(self):
+        """`launch` as one graph launch (no host sync)."""
+        self._reset()
+        self.capture_graph().launch(self.prog.ctx)
+
+    def _enqueue(self, events=None):
         st, model, dt, prog = self._st, self.model, self.dt, self.prog
         ctx = prog.ctx
         T, n, R, seed, base, off, b2 = (st[k] for k in ("T", "n", "R", "seed", "base", "off", "b2"))
-        if self._launched:
-            # a repeated run starts from the same initial panel
-            for o, arr in self._init_state:
-                nat.h2d(ctx, base + o, arr)
-        self._launched = True
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
         # EV rows for every (period, regime) with a stateful continuation

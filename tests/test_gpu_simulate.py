@@ -268,3 +268,93 @@ def test_simulate_dcegm_model_f64(native):
     ok = alive & np.isfinite(df["value"])
     assert ok.sum() > 0
     assert (df.loc[ok, "consumption"] <= df.loc[ok, "wealth"] + 1e-12).all()
+
+
+def test_simulate_stochastic_f32_stepwise_against_oracle(native):
+    """The float32 production path against the FP64 oracle, one period at a
+    time from the device's own panel (so one near-tie does not send the rest
+    of a path elsewhere): at every simulated (agent, period) the chosen action
+    is feasible and is the oracle's first argmax of Q at the agent's state, or
+    a near-tie within 8 float32 ulps of the maximum (flagged, reported);
+    the recorded value is within 1e-5 relative of the oracle's maximum."""
+    T = 6
+    m = ex.retirement_model(T, n_wealth=60, n_consumption=80, income_nodes=5)
+    inc = {"rho": 0.9, "sigma": 0.15, "mu": 0.0}
+    p = ex.retirement_params(T, income=inc)
+    rng = np.random.default_rng(12)
+    n = 160
+    ic = {"wealth": rng.uniform(1, 400, n), "income": rng.normal(0, 0.3, n),
+          "regime_id": np.zeros(n, dtype=int), "age": np.zeros(n)}
+    V, _ = m.solve(params=p, log_level="off", dtype=np.float32, return_policy=True)
+    res = m.simulate(params=p, initial_conditions=ic, period_to_regime_to_V_arr=V, log_level="off", seed=77,
+                     dtype=np.float32)
+    raw = res.raw
+    V64 = {t: {r: np.asarray(v, dtype=np.float64) for r, v in rr.items()} for t, rr in V.items()}
+    names = list(m.regime_id_class.__categories__)
+    R = {nm: og._R(nm, m.regimes[nm], np.float32, p, m.fixed_params) for nm in names}
+    checked = ties = 0
+    worst = 0.0
+    for t in range(T):
+        for k, r in enumerate(names):
+            idx = np.nonzero((raw["regime"][t] == k) & (raw["action"][t] >= 0))[0]
+            if idx.size == 0:
+                continue
+            reg = R[r]
+            w = raw["w"][t, idx].astype(np.float64)
+            z = raw["z"][t, idx].astype(np.float64)
+            sdd = raw["sdd"][t, idx].astype(np.int64)
+            Q, F = osim._regime_period(m, R, reg, t, p, m.fixed_params, np.dtype(np.float32), V64, w, z, sdd,
+                                       agents=idx, seed=77, return_q=True)
+            Qa = np.where(F, Q, -np.inf).reshape(idx.size, -1)
+            Fa = np.broadcast_to(F, Q.shape).reshape(idx.size, -1)
+            a_o = np.argmax(Qa, axis=1)
+            q_max = Qa[np.arange(idx.size), a_o]
+            a_g = raw["action"][t, idx]
+            assert np.all(Fa[np.arange(idx.size), a_g]), (t, r)
+            q_g = Qa[np.arange(idx.size), a_g]
+            ulp = np.spacing(np.abs(q_max).astype(np.float32)).astype(np.float64)
+            near = (a_g != a_o) & (q_max - q_g <= 8 * ulp)
+            assert np.all((a_g == a_o) | near), (t, r, int(((a_g != a_o) & ~near).sum()))
+            ties += int(near.sum())
+            fin = np.isfinite(q_max)
+            v_g = raw["value"][t, idx].astype(np.float64)
+            assert np.array_equal(np.isfinite(v_g), fin)
+            if fin.any():
+                worst = max(worst, float(np.max(np.abs(v_g[fin] - q_max[fin]) / np.abs(q_max[fin]))))
+            checked += idx.size
+    print(f"{checked} agent-periods checked, {ties} flagged near-ties, max rel value err {worst:.2e}")
+    assert checked > n * (T - 2)
+    assert worst <= 1e-5
+
+
+def test_simulation_graph_replays_equal_eager(native):
+    """The simulation captured as a HIP graph (EV launch + every period's
+    kernels) gives the eager panel on every replay, with eager runs between."""
+    from pylcm_amd._build import resolve_grids
+    from pylcm_amd._gs_program import GsProgram
+    from pylcm_amd._simulate import SimProgram, _initial_conditions
+
+    T = 8
+    m = ex.retirement_model(T, n_wealth=120, n_consumption=120, income_nodes=7)
+    p = ex.retirement_params(T, income={"rho": 0.95, "sigma": 0.1, "mu": 0.0})
+    prog = GsProgram(m, [p], np.float32)
+    prog.run(use_graph=False)
+    rg = resolve_grids(m.layouts, p, m.fixed_params, np.float32)
+    nodes = np.asarray(rg.proc[("working_life", "income")][0], dtype=np.float64)
+    rng = np.random.default_rng(8)
+    n = 5000
+    ic = _initial_conditions(m, {"wealth": rng.uniform(1, 400, n), "income": nodes[rng.integers(0, 7, n)],
+                                 "regime_id": np.zeros(n, dtype=int)})
+    sim = SimProgram(m, p, prog)
+    sim.prepare(ic, 99)
+    sim.launch()
+    prog.ctx.sync()
+    eager = sim.collect(ic).raw
+    for _ in range(3):
+        sim.launch_graph()
+        prog.ctx.sync()
+        got = sim.collect(ic).raw
+        for k in eager:
+            assert np.array_equal(got[k], eager[k], equal_nan=True), k
+        sim.launch()  # eager work between the replays
+        prog.ctx.sync()
