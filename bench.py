@@ -380,13 +380,19 @@ def _serve_window(args, rate, seconds, seed):
     from pylcm_amd import examples as ex
     from pylcm_amd.serving import SolveServer, serve_poisson
 
-    server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=4,
-                         lower_procs=args.serve_procs)
+    import sys as _sys
+
+    old_si = _sys.getswitchinterval()
+    if args.serve_switch_ms > 0:
+        _sys.setswitchinterval(args.serve_switch_ms * 1e-3)
+    server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=args.serve_threads,
+                         lower_procs=args.serve_procs, lanes=args.serve_lanes, queue_depth=args.serve_depth)
     make = ex.serving_request_factory(seed=7)
     t0 = time.perf_counter()
     reqs = serve_poisson(server, make, rate=rate, duration=seconds, seed=seed)
     wall = time.perf_counter() - t0
     server.close()
+    _sys.setswitchinterval(old_si)
     ok = [r for r in reqs if r.error is None]
     lat = np.array([r.latency for r in ok])
     span = max(r.t_done for r in ok) - min(r.t_submit for r in ok)
@@ -398,7 +404,8 @@ def _serve_window(args, rate, seconds, seed):
            "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
            "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
            "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
-           "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())}}
+           "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())},
+           "lanes": len(server.lanes)}
     # where the latency goes: lowering (worker processes), placement (device
     # memory wait + uploads), and the solve (admission to completion)
     stages = {"lower": [r.t_lowered - r.t_submit for r in ok],
@@ -406,6 +413,18 @@ def _serve_window(args, rate, seconds, seed):
               "solve": [r.t_done - r.t_admit for r in ok]}
     out["stage_ms"] = {k: {"p50": round(1e3 * float(np.percentile(v, 50)), 2),
                            "p99": round(1e3 * float(np.percentile(v, 99)), 2)} for k, v in stages.items()}
+    # capacities of the three resources a request passes through, in
+    # requests/s: the lowering processes (their mean time per request in the
+    # worker), the device (fused-launch busy time per request), and the
+    # serving loop's own host time (admit + launch + retire per request)
+    wk = np.array([r.worker_s for r in ok if r.worker_s > 0])
+    busy = out["gpu_busy_s"]
+    loop_s = sum(server.timing.values())
+    out["capacity_req_per_s"] = {
+        "lowering": round(args.serve_procs / float(wk.mean()), 1) if wk.size else None,
+        "lowering_worker_ms_mean": round(1e3 * float(wk.mean()), 2) if wk.size else None,
+        "device": round(len(ok) / busy, 1) if busy > 0 else None,
+        "serving_loop": round(len(ok) / loop_s, 1) if loop_s > 0 else None}
     log(f"serving at {rate:g}/s: {len(ok)}/{len(reqs)} requests in {wall:.1f}s, {out['solves_per_s']:.1f} solves/s, "
         f"p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {server.launches} fused launches")
     return out
@@ -537,6 +556,11 @@ def main():
     ap.add_argument("--serve-procs", type=int, default=12)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
     ap.add_argument("--serve-tile", type=int, default=0, help="0: chosen per fused launch")
+    ap.add_argument("--serve-lanes", type=int, default=1, help="streams carrying the fused launches")
+    ap.add_argument("--serve-threads", type=int, default=4, help="threads placing lowered requests")
+    ap.add_argument("--serve-depth", type=int, default=2, help="fused launches queued per lane")
+    ap.add_argument("--serve-switch-ms", type=float, default=0.0,
+                    help="interpreter thread switch interval during serving (0: unchanged)")
     ap.add_argument("--fresh-batches", type=int, default=4)
     ap.add_argument("--fresh-procs", type=int, default=14)
     args = ap.parse_args()

@@ -331,6 +331,25 @@ int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                      const LcmGsTask* d_tasks, int n_tasks, int tile,
                      size_t lds_bytes, int variant);
 
+/*
+ * One fused launch of a serving loop in a single call, all on ctx's stream:
+ * record ev_start; copy n_tasks task records from pinned host memory h_tasks
+ * to d_tasks; lcm_gs_launch_ex; record ev_ring; then n_copies device-to-host
+ * copies (src[i] -> dst[i], bytes[i]; dst pinned) of results that this launch
+ * completes; record ev_end. Events may be null (not recorded). No counterpart
+ * in the reference, which has no server: it is for host loops that issue many
+ * small launches, which then pay one foreign call per launch instead of six.
+ */
+int lcm_gs_serve_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs, const LcmGsTask* h_tasks,
+                        LcmGsTask* d_tasks, int n_tasks, int tile, size_t lds_bytes, int variant,
+                        void* ev_start, void* ev_ring, void* const* dst, const void* const* src,
+                        const size_t* bytes, int n_copies, void* ev_end);
+
+/* Milliseconds from event ref to each of evs[0..n) (all recorded and
+ * complete), in one call; for the busy-time accounting of a serving loop (no
+ * counterpart in the reference). */
+int lcm_event_elapsed_batch(void* ref, void* const* evs, int n, float* out_ms);
+
 /* ------------------------------------------------------------------------- */
 /* DC-EGM with the FUES upper envelope (all arithmetic FP64)                  */
 /* ------------------------------------------------------------------------- */

@@ -187,11 +187,22 @@ def lower_problem(model, params, dtype):
     return tabs, rg
 
 
+_LDS_CACHE: dict = {}
+
+
+def _lds_bytes(code: int, g: int, k: int) -> int:
+    key = (code, g, k)
+    b = _LDS_CACHE.get(key)
+    if b is None:
+        b = _LDS_CACHE[key] = int(nat.load_library().lcm_gs_lds_bytes(code, g, k))
+    return b
+
+
 def step_lds_bytes(dtype, gmax, kmax):
-    """`lcm_gs_lds_bytes` for each step's largest next grid and constraint count."""
+    """`lcm_gs_lds_bytes` for each step's largest next grid and constraint count
+    (asked of the library once per distinct pair)."""
     code = nat.LCM_DTYPE_F64 if np.dtype(dtype) == np.float64 else nat.LCM_DTYPE_F32
-    L = nat.load_library()
-    out = [int(L.lcm_gs_lds_bytes(code, int(g), int(k))) for g, k in zip(gmax, kmax)]
+    out = [_lds_bytes(code, int(g), int(k)) for g, k in zip(gmax, kmax)]
     for b, g, k in zip(out, gmax, kmax):
         if b > LDS_LIMIT:
             raise UnsupportedModelError(

@@ -164,6 +164,8 @@ _SIGS = {
     "lcm_memset_async": ([vp, vp, C.c_int, C.c_size_t], C.c_int),
     "lcm_gs_launch": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_size_t], C.c_int),
     "lcm_gs_launch_ex": ([vp, C.c_int, vp, vp, C.c_int, C.c_int, C.c_size_t, C.c_int], C.c_int),
+    "lcm_gs_serve_launch": ([vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_size_t, C.c_int, vp, vp,
+                             C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_size_t), C.c_int, vp], C.c_int),
     "lcm_graph_begin": ([vp], C.c_int),
     "lcm_graph_end": ([vp, C.POINTER(vp)], C.c_int),
     "lcm_graph_launch": ([vp, vp], C.c_int),
@@ -174,6 +176,7 @@ _SIGS = {
     "lcm_event_elapsed": ([vp, vp, C.POINTER(C.c_float)], C.c_int),
     "lcm_event_query": ([vp], C.c_int),
     "lcm_event_synchronize": ([vp], C.c_int),
+    "lcm_event_elapsed_batch": ([vp, C.POINTER(vp), C.c_int, C.POINTER(C.c_float)], C.c_int),
     "lcm_copy_launch": ([vp, vp, vp, C.c_size_t], C.c_int),
     "lcm_abi_sizes": ([C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
     "lcm_gs_lds_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),

@@ -37,15 +37,22 @@ class AgeGrid:
             self._step = None
         elif start is not None and stop is not None and step is not None:
             st = _parse_step(step)
-            span = Fraction(stop) - Fraction(start)
-            if span < 0 or span % st != 0:
-                raise GridInitializationError("step must divide stop - start evenly")
-            n = int(span // st) + 1
-            vals = tuple(Fraction(start) + i * st for i in range(n))
+            if all(isinstance(x, (int, np.integer)) for x in (start, stop, st)):
+                # integer ages: plain integer arithmetic (exact, and cheap)
+                span = int(stop) - int(start)
+                if span < 0 or span % st != 0:
+                    raise GridInitializationError("step must divide stop - start evenly")
+                vals = tuple(range(int(start), int(stop) + 1, st))
+            else:
+                span = Fraction(stop) - Fraction(start)
+                if span < 0 or span % st != 0:
+                    raise GridInitializationError("step must divide stop - start evenly")
+                n = int(span // st) + 1
+                vals = tuple(Fraction(start) + i * st for i in range(n))
             self._step = st
         else:
             raise GridInitializationError("Must specify 'start/stop/step' or 'exact_values'.")
-        self._is_integer = all(Fraction(v).denominator == 1 for v in vals)
+        self._is_integer = all(isinstance(v, (int, np.integer)) or Fraction(v).denominator == 1 for v in vals)
         if self._is_integer:
             self._exact = tuple(int(v) for v in vals)
         else:

@@ -919,3 +919,24 @@ extern "C" int lcm_gs_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                              size_t lds_bytes) {
   return lcm_gs_launch_ex(ctx, dtype, d_descs, d_tasks, n_tasks, tile, lds_bytes, lcm::kVarWdep);
 }
+
+extern "C" int lcm_gs_serve_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs, const LcmGsTask* h_tasks,
+                                   LcmGsTask* d_tasks, int n_tasks, int tile, size_t lds_bytes, int variant,
+                                   void* ev_start, void* ev_ring, void* const* dst, const void* const* src,
+                                   const size_t* bytes, int n_copies, void* ev_end) {
+  using namespace lcm;
+  if (!ctx || !h_tasks || !d_tasks || (n_copies > 0 && (!dst || !src || !bytes)) || n_copies < 0)
+    return fail(LCM_E_ARG, "lcm_gs_serve_launch: null argument");
+  if (ev_start) LCM_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev_start), ctx->stream));
+  LCM_HIP(hipMemcpyAsync(d_tasks, h_tasks, size_t(n_tasks) * sizeof(LcmGsTask), hipMemcpyHostToDevice,
+                         ctx->stream));
+  const int rc = lcm_gs_launch_ex(ctx, dtype, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant);
+  if (rc != LCM_OK) return rc;
+  if (ev_ring) LCM_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev_ring), ctx->stream));
+  for (int i = 0; i < n_copies; ++i) {
+    if (!dst[i] || !src[i]) return fail(LCM_E_ARG, "lcm_gs_serve_launch: null copy pointer");
+    if (bytes[i]) LCM_HIP(hipMemcpyAsync(dst[i], src[i], bytes[i], hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (ev_end) LCM_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(ev_end), ctx->stream));
+  return LCM_OK;
+}

@@ -209,6 +209,16 @@ int lcm_event_record(lcm_ctx* ctx, void* ev) {
   return LCM_OK;
 }
 
+int lcm_event_elapsed_batch(void* ref, void* const* evs, int n, float* out_ms) {
+  if (!ref || (n > 0 && (!evs || !out_ms)) || n < 0) return fail(LCM_E_ARG, "lcm_event_elapsed_batch: null argument");
+  for (int i = 0; i < n; ++i) {
+    if (!evs[i]) return fail(LCM_E_ARG, "lcm_event_elapsed_batch: null event");
+    LCM_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(evs[i])));
+    LCM_HIP(hipEventElapsedTime(&out_ms[i], reinterpret_cast<hipEvent_t>(ref), reinterpret_cast<hipEvent_t>(evs[i])));
+  }
+  return LCM_OK;
+}
+
 int lcm_event_query(void* ev) {
   if (!ev) return fail(LCM_E_ARG, "lcm_event_query: null event");
   const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(ev));

@@ -21,9 +21,12 @@ bit equality).
 from __future__ import annotations
 
 import ctypes as C
+import os
 import queue
+import tempfile
 import threading
 import time
+import uuid
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -35,24 +38,147 @@ from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_probl
 ALIGN = 256
 
 
-def _lower_in_worker(blob: bytes, dtype: str, tile: int) -> dict:
+def _lower_in_worker(blob: bytes, dtype: str, tile: int, spill: str | None = None) -> dict:
     """Lower and pack one request in a worker process (no device code is
-    loaded there): everything the server needs to place and run it."""
+    loaded there): everything the server needs to place and run it. With
+    `spill` (a path prefix, normally in /dev/shm) the bulky arrays (the table
+    image, the descriptor records and the task rows) are written to a file of
+    that prefix instead of travelling back pickled through the pool's pipe,
+    whose single receiving thread in the server then only unpickles a few
+    kilobytes per request."""
     import cloudpickle
 
     from pylcm_amd._batch import pack_problem
 
+    t0 = time.perf_counter()
     model, params = cloudpickle.loads(blob)
-    return pack_problem(model, params, dtype, tile)
+    data = pack_problem(model, params, dtype, tile)
+    if spill is not None:
+        img, D0, cat = data.pop("image"), data.pop("D0"), data.pop("tasks_cat")
+        data.pop("tasks_host")
+        path = f"{spill}{os.getpid()}_{time.perf_counter_ns()}"
+        with open(path, "wb") as f:
+            f.write(img.tobytes())
+            f.write(D0.tobytes())
+            f.write(np.ascontiguousarray(cat, dtype=np.int32).tobytes())
+        data["spilled"] = (path, int(img.nbytes), len(D0), int(cat.shape[0]))
+    data["worker_s"] = time.perf_counter() - t0
+    return data
 
 
-def _warm_worker() -> int:
-    """Import what `_lower_in_worker` needs, so the first request pays no start-up."""
-    import cloudpickle  # noqa: F401
+def _unspill(data: dict) -> np.ndarray:
+    """Read back what `_lower_in_worker` spilled: fills data's D0 and
+    tasks_host (copies) and returns the table image (a copy in host memory);
+    removes the file."""
+    path, n_img, n_desc, n_task = data.pop("spilled")
+    size = n_img + n_desc * DESC_DTYPE.itemsize + n_task * 12
+    buf = np.empty(size, dtype=np.uint8)
+    try:
+        with open(path, "rb", buffering=0) as f:
+            got = f.readinto(memoryview(buf))
+    finally:
+        os.unlink(path)
+    if got != size:
+        raise RuntimeError(f"request spill file {path}: {got} of {size} bytes")
+    data["D0"] = buf[n_img: n_img + n_desc * DESC_DTYPE.itemsize].view(DESC_DTYPE)
+    cat = buf[n_img + n_desc * DESC_DTYPE.itemsize:].view(np.int32).reshape(n_task, 3)
+    counts = np.asarray(data["task_counts"], dtype=np.int64)
+    data["tasks_host"] = np.split(cat, np.cumsum(counts)[:-1])
+    return buf[:n_img]
+
+
+def _pool_worker_main(conn, spill):
+    """A lowering process: receive (rid, dtype, tile, blob), send back
+    (rid, data) or (rid, exception), until told to stop (None)."""
+    import cloudpickle  # noqa: F401  (imported once, before the first request)
 
     from pylcm_amd import examples  # noqa: F401
 
-    return 0
+    conn.send(("ready", None))
+    while True:
+        try:
+            msg = conn.recv()
+        except EOFError:
+            return
+        if msg is None:
+            return
+        rid, dtype, tile, blob = msg
+        try:
+            out = (rid, _lower_in_worker(blob, dtype, tile, spill))
+        except BaseException as e:  # surfaced to the client of this request
+            out = (rid, e)
+        conn.send(out)
+
+
+class _LoweringPool:
+    """Worker processes fed through one pipe each. The server's submitting
+    thread writes each request to the worker with the fewest outstanding
+    ones; one receiving thread waits on all pipes and hands each result to
+    `on_result`. Compared with a process-pool executor there is no
+    management thread with a shallow call queue between the two, so a busy
+    interpreter cannot leave the workers idle."""
+
+    def __init__(self, procs: int, spill: str, on_result):
+        import multiprocessing
+
+        ctx = multiprocessing.get_context("spawn")
+        self.conns, self.procs, self.outstanding = [], [], []
+        for _ in range(int(procs)):
+            a, b = ctx.Pipe()
+            pr = ctx.Process(target=_pool_worker_main, args=(b, spill), daemon=True)
+            pr.start()
+            b.close()
+            self.conns.append(a)
+            self.procs.append(pr)
+            self.outstanding.append(set())
+        for c in self.conns:  # every worker has imported what it needs
+            tag, _ = c.recv()
+            assert tag == "ready"
+        self.on_result = on_result
+        self._lock = threading.Lock()
+        self._stop = False
+        self._rx = threading.Thread(target=self._receive, daemon=True)
+        self._rx.start()
+
+    def submit(self, rid: int, dtype: str, tile: int, blob: bytes):
+        with self._lock:
+            k = min(range(len(self.conns)), key=lambda j: len(self.outstanding[j]))
+            self.outstanding[k].add(rid)
+        self.conns[k].send((rid, dtype, tile, blob))
+
+    def _receive(self):
+        from multiprocessing.connection import wait
+
+        live = list(self.conns)
+        while live and not self._stop:
+            for c in wait(live, timeout=0.1):
+                k = self.conns.index(c)
+                try:
+                    rid, out = c.recv()
+                except (EOFError, OSError):
+                    # the worker ended: every request it still held fails
+                    live.remove(c)
+                    with self._lock:
+                        lost, self.outstanding[k] = self.outstanding[k], set()
+                    for rid in lost:
+                        self.on_result(rid, RuntimeError(f"lowering process {k} ended"), k)
+                    continue
+                with self._lock:
+                    self.outstanding[k].discard(rid)
+                self.on_result(rid, out, k)
+
+    def close(self):
+        self._stop = True
+        for c in self.conns:
+            try:
+                c.send(None)
+            except (OSError, BrokenPipeError):
+                pass
+        for pr in self.procs:
+            pr.join(timeout=5)
+            if pr.is_alive():
+                pr.terminate()
+        self._rx.join(timeout=1)
 
 
 class _PackedRequest:
@@ -110,13 +236,16 @@ class Request:
     prog: GsProgram | None = None
     error: BaseException | None = None
     t_lowered: float = 0.0
+    worker_s: float = 0.0  # lowering time inside the worker process
+    staged: list | None = None  # period-0 V copied to pinned host memory by the stream
+    n_act_steps: list | None = None  # most actions per state of each step's tasks
     t_admit: float = 0.0
     t_done: float = 0.0
     slab: tuple | None = None
     desc: tuple | None = None
     tasks: list | None = None
     desc_base: int = 0
-    future: object = None
+    lowered: object = None  # a worker's result (dict) or its exception
     result: dict | None = None
     done: threading.Event = field(default_factory=threading.Event)
 
@@ -137,13 +266,15 @@ class _Cohort:
         self._ending = {}
         for r in members:
             self._ending.setdefault(r.prog.n_steps - 1, []).append(r)
+        empty = np.zeros(0, dtype=TASK_DTYPE)
         for s in range(self.n_steps):
-            parts = [r.tasks[s] for r in members if s < r.prog.n_steps and len(r.tasks[s])]
-            self.steps.append(np.concatenate(parts) if parts else np.zeros(0, dtype=TASK_DTYPE))
-            arr = self.steps[-1]
-            self.n_actions.append(int(n_act_of_desc[arr["desc"]].max()) if len(arr) else 1)
-            self.lds.append(max([r.prog.lds_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
-            self.variant.append(max([r.prog.wdep_by_step[s] for r in members if s < r.prog.n_steps] or [0]))
+            live = [r for r in members if s < r.prog.n_steps]
+            parts = [r.tasks[s] for r in live if len(r.tasks[s])]
+            self.steps.append(parts[0] if len(parts) == 1 else (np.concatenate(parts) if parts else empty))
+            # per-request figures computed when each request was placed
+            self.n_actions.append(max([r.n_act_steps[s] for r in live] or [1]))
+            self.lds.append(max([r.prog.lds_by_step[s] for r in live] or [0]))
+            self.variant.append(max([r.prog.wdep_by_step[s] for r in live] or [0]))
         self.s = 0
 
     @property
@@ -152,6 +283,49 @@ class _Cohort:
 
     def finished_at(self, s: int) -> list:
         return self._ending.get(s, [])
+
+
+class _Lane:
+    """One stream of fused launches: its cohorts, its ring of task buffers and
+    the events that order retirement. Lanes run concurrently on the device,
+    so one lane's launch fills the CUs that another's slowest workgroups
+    leave idle at the end of its launch."""
+
+    def __init__(self, n_ring: int, max_tasks: int):
+        self.ctx = nat.Context()
+        self.n_ring = n_ring
+        self.task_host = [nat.PinnedBuffer(max_tasks * TASK_DTYPE.itemsize) for _ in range(n_ring)]
+        self.task_dev = [nat.DeviceBuffer(max_tasks * TASK_DTYPE.itemsize) for _ in range(n_ring)]
+        self.ring_events = [nat.Event() for _ in range(n_ring)]
+        self.ring_used = [False] * n_ring
+        self.ring_i = 0
+        self.inflight: list[_Cohort] = []  # cohorts with steps left to launch
+        self.finishing: list = []           # (event, requests) in stream order
+        # (start, end) events of the last TIMING launches, reused in a ring;
+        # each half is read back in one call once the other half is being
+        # recorded (its launches have long finished)
+        self.tev = [(nat.Event(), nat.Event()) for _ in range(self.TIMING)]
+        self.tev_i = 0
+        self.tev_live = [False] * self.TIMING
+        self.intervals: list[tuple[float, float]] = []  # ms from the server's reference event
+
+    TIMING = 256
+
+    def harvest(self, ref, lo: int, hi: int):
+        """Read back the recorded (start, end) pairs of ring slots [lo, hi)."""
+        idx = [j for j in range(lo, hi) if self.tev_live[j]]
+        if not idx:
+            return
+        evs = (C.c_void_p * (2 * len(idx)))(*[e.handle.value for j in idx for e in self.tev[j]])
+        out = (C.c_float * (2 * len(idx)))()
+        nat.call("lcm_event_elapsed_batch", ref.handle, evs, 2 * len(idx), out)
+        self.intervals += [(out[2 * i], out[2 * i + 1]) for i in range(len(idx))]
+        for j in idx:
+            self.tev_live[j] = False
+
+    @property
+    def load(self) -> int:
+        return sum(c.remaining for c in self.inflight)
 
 
 class SolveServer:
@@ -170,6 +344,9 @@ class SolveServer:
     more workgroups). The tile does not change results: the maximum and its
     first maximiser are merged the same way whatever the split. `max_inflight`
     caps admitted requests.
+    `lanes` streams carry the fused launches: each admitted cohort joins the
+    lane with the fewest requests in flight, and the lanes' launches overlap
+    on the device.
     `lower_procs` > 0 lowers requests in that many spawned worker processes
     (models are sent with cloudpickle); as with any spawned pool, the main
     script must guard its entry point with `if __name__ == "__main__":`.
@@ -178,7 +355,8 @@ class SolveServer:
     def __init__(self, *, dtype=np.float32, tile: int = 0, budget_bytes: int | None = None,
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_threads: int = 1,
-                 lower_procs: int = 0, tile_table: dict | None = None):
+                 lower_procs: int = 0, tile_table: dict | None = None, lanes: int = 1,
+                 result_pinned_bytes: int = 64 << 20, queue_depth: int = 2):
         self.dt = np.dtype(dtype)
         self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         self.auto_tile = int(tile) == 0
@@ -186,7 +364,6 @@ class SolveServer:
         self.tile_table = tile_table  # None: the committed table
         self.tiles_used = {}  # tile -> fused launches
         self.fetch_mode = fetch
-        self.ctx = nat.Context()        # kernels and task uploads
         self.copy_ctx = nat.Context()   # request images in, results out
         _, free, _ = nat.device_info()
         budget = int(free * free_fraction)
@@ -201,39 +378,41 @@ class SolveServer:
         self.max_inflight = int(max_inflight)
         self.n_w_of_desc = np.zeros(int(max_desc), dtype=np.int32)  # for splitting tasks into finer tiles
         self.n_act_of_desc = np.ones(int(max_desc), dtype=np.int64)  # actions per state, for the tile choice
-        # at most two fused launches queued on the stream: while one runs, the
-        # next one gathers every step that becomes ready, so launches stay large
-        self.n_ring = 2
-        self.task_host = [nat.PinnedBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
-        self.task_dev = [nat.DeviceBuffer(self.max_tasks * TASK_DTYPE.itemsize) for _ in range(self.n_ring)]
-        self.ring_events = [nat.Event() for _ in range(self.n_ring)]
-        self.ring_used = [False] * self.n_ring
-        self.ring_i = 0
+        # per lane at most `queue_depth` fused launches queued on its stream:
+        # while one runs, the next one gathers every step that becomes ready,
+        # so launches stay large
+        self.lanes = [_Lane(int(queue_depth), self.max_tasks) for _ in range(max(1, int(lanes)))]
+        # fetch="first": each request's period-0 V is copied by its lane's
+        # stream, right after its last launch, into pinned host memory, so
+        # retiring it is a host copy (requests that do not fit fall back to a
+        # synchronous copy)
+        self.res_slab = _Slab(int(result_pinned_bytes)) if fetch == "first" else None
+        self.res_buf = nat.PinnedBuffer(self.res_slab.size) if self.res_slab else None
+        self.ctx = self.lanes[0].ctx
+        self._t0_event = nat.Event()
+        self._t0_event.record(self.ctx)
         self._lowq: queue.Queue = queue.Queue()
         self._ready: list[Request] = []      # placed on the device, not yet in a launch
         self._ready_lock = threading.Lock()
         self._mem_cv = threading.Condition()  # guards the slabs; notified on retirement
         self._n_placed = 0
-        self.inflight: list[_Cohort] = []   # cohorts with steps left to launch
-        self.finishing: list = []            # (event, requests) in stream order
         self.completed: list[Request] = []
         self._next_id = 0
         self.launches = 0
         self.launch_tasks = 0
         self.timing = {"admit": 0.0, "launch": 0.0, "retire": 0.0}
-        self._launch_events = []  # (start, end) per fused launch, for the GPU busy time
         self._stop = False
         # lower_procs > 0: lowering runs in that many worker processes (the
         # model travels by cloudpickle) and the threads only place requests
         self._pool = None
+        self._spill = None
         if lower_procs > 0:
-            import multiprocessing
-            from concurrent.futures import ProcessPoolExecutor
-
-            self._pool = ProcessPoolExecutor(max_workers=int(lower_procs),
-                                             mp_context=multiprocessing.get_context("spawn"))
-            for f in [self._pool.submit(_warm_worker) for _ in range(int(lower_procs))]:
-                f.result()
+            # worker results' bulky arrays travel through files of this prefix
+            # (memory-backed /dev/shm when present); close() removes leftovers
+            base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
+            self._spill = os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
+            self._by_rid: dict[int, Request] = {}
+            self._pool = _LoweringPool(int(lower_procs), self._spill, self._on_lowered)
             lower_threads = max(lower_threads, 2)
         self._workers = [threading.Thread(target=self._lower_worker, daemon=True) for _ in range(lower_threads)]
         for w in self._workers:
@@ -246,10 +425,20 @@ class SolveServer:
         if self._pool is not None:
             import cloudpickle
 
-            req.future = self._pool.submit(_lower_in_worker, cloudpickle.dumps((model, params)),
-                                           self.dt.str, self.tile)
-        self._lowq.put(req)
+            with self._ready_lock:
+                self._by_rid[req.rid] = req
+            self._pool.submit(req.rid, self.dt.str, self.tile, cloudpickle.dumps((model, params)))
+        else:
+            self._lowq.put(req)
         return req
+
+    def _on_lowered(self, rid, out, worker: int):
+        """A worker's result or exception (receiving thread): queue the
+        request for placement, in completion order."""
+        with self._ready_lock:
+            req = self._by_rid.pop(rid)
+        req.lowered = out
+        self._lowq.put(req)
 
     def close(self):
         self._stop = True
@@ -258,7 +447,15 @@ class SolveServer:
         with self._mem_cv:
             self._mem_cv.notify_all()
         if self._pool is not None:
-            self._pool.shutdown(wait=False, cancel_futures=True)
+            self._pool.close()
+        if self._spill is not None:
+            import glob
+
+            for path in glob.glob(self._spill + "*"):
+                try:
+                    os.unlink(path)
+                except OSError:
+                    pass
 
     # -- host lowering and admission (worker threads) ------------------------------
     def _lower_worker(self):
@@ -268,11 +465,14 @@ class SolveServer:
             if req is None:
                 return
             try:
-                if req.future is not None:
-                    data = req.future.result()
-                    req.future = None
+                if req.lowered is not None:
+                    data, req.lowered = req.lowered, None
+                    if isinstance(data, BaseException):
+                        raise data
+                    req.worker_s = data.get("worker_s", 0.0)
+                    image = _unspill(data) if "spilled" in data else data["image"]
                     prog = _PackedRequest(data, self.dt)
-                    D0, masks, image = data["D0"], data["masks"], data["image"]
+                    D0, masks = data["D0"], data["masks"]
                 else:
                     low = lower_problem(req.model, req.params, self.dt)
                     prog = GsProgram.of([(req.model, req.params)], self.dt, lowered=[low], tile=self.tile,
@@ -316,18 +516,28 @@ class SolveServer:
         D = GsProgram.records_at(D0, masks, base)
         nat.h2d(copy_ctx, self.desc_buf.ptr + ds[0], D)
         desc_base = ds[0] // DESC_DTYPE.itemsize
-        steps = []
-        for t in prog.tasks_host:
-            rec = np.zeros(len(t), dtype=TASK_DTYPE)
-            if len(t):
-                rec["desc"] = t[:, 0] + desc_base
-                rec["sd"] = t[:, 1]
-                rec["w0"] = t[:, 2]
-            steps.append(rec)
-        req.tasks = steps
+        # every step's task records from one concatenated array (a handful of
+        # numpy calls per request, whatever its horizon), then split per step
+        counts = np.fromiter((len(t) for t in prog.tasks_host), dtype=np.int64, count=len(prog.tasks_host))
+        cat = (np.concatenate([t for t in prog.tasks_host if len(t)]) if counts.any()
+               else np.zeros((0, 3), dtype=np.int32))
+        rec = np.zeros(len(cat), dtype=TASK_DTYPE)
+        rec["desc"] = cat[:, 0] + desc_base
+        rec["sd"] = cat[:, 1]
+        rec["w0"] = cat[:, 2]
+        ends = np.cumsum(counts)
+        req.tasks = np.split(rec, ends[:-1])
         req.desc_base = desc_base
         self.n_w_of_desc[desc_base: desc_base + prog.n_desc] = D0["n_w"]
-        self.n_act_of_desc[desc_base: desc_base + prog.n_desc] = D0["n_d"].astype(np.int64) * D0["n_c"]
+        n_act = D0["n_d"].astype(np.int64) * D0["n_c"]
+        self.n_act_of_desc[desc_base: desc_base + prog.n_desc] = n_act
+        # most actions per state of each step's tasks
+        act = n_act[cat[:, 0]] if len(cat) else np.zeros(0, dtype=np.int64)
+        nz = counts > 0
+        steps_max = np.ones(len(counts), dtype=np.int64)
+        if nz.any():
+            steps_max[nz] = np.maximum.reduceat(act, (ends - counts)[nz])
+        req.n_act_steps = steps_max.tolist()
         req.slab, req.desc = sl, ds
         req.t_admit = time.perf_counter()
         with self._ready_lock:
@@ -338,20 +548,30 @@ class SolveServer:
         with self._ready_lock:
             ready, self._ready = self._ready, []
         if ready:
-            self.inflight.append(_Cohort(ready, self.n_act_of_desc))
+            lane = min(self.lanes, key=lambda ln: ln.load)
+            lane.inflight.append(_Cohort(ready, self.n_act_of_desc))
+
+    @property
+    def inflight(self) -> list:
+        return [c for ln in self.lanes for c in ln.inflight]
+
+    @property
+    def finishing(self) -> list:
+        return [f for ln in self.lanes for f in ln.finishing]
 
     def _inflight_requests(self) -> int:
-        return sum(c.remaining for c in self.inflight)
+        return sum(ln.load for ln in self.lanes)
 
-    def _launch(self):
-        """One fused launch: the next backward step of every request in flight
-        (per cohort, one precomputed task array)."""
-        k = self.ring_i
-        if self.ring_used[k]:
-            self.ring_events[k].synchronize()  # the copy from this host slot has finished
-        rec = np.frombuffer(self.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
+    def _launch(self, lane: _Lane | None = None):
+        """One fused launch on a lane: the next backward step of every request
+        in flight there (per cohort, one precomputed task array)."""
+        lane = lane or self.lanes[0]
+        k = lane.ring_i
+        if lane.ring_used[k]:
+            lane.ring_events[k].synchronize()  # the copy from this host slot has finished
+        rec = np.frombuffer(lane.task_host[k].array, dtype=TASK_DTYPE, count=self.max_tasks)
         picks, n_coarse, lds, variant, n_act = [], 0, 0, 0, 1
-        for c in self.inflight:
+        for c in lane.inflight:
             arr = c.steps[c.s]
             if n_coarse + len(arr) + 8 > self.max_tasks:
                 break  # the rest join the next launch
@@ -364,13 +584,13 @@ class SolveServer:
             return False
         if n_coarse == 0:  # nothing to run this step (no regime of these cohorts is active)
             ev = nat.Event()
-            ev.record(self.ctx)
+            ev.record(lane.ctx)
             done = [r for c in picks for r in c.finished_at(c.s)]
             if done:
-                self.finishing.append((ev, done))
+                lane.finishing.append((ev, done))
             for c in picks:
                 c.s += 1
-            self.inflight = [c for c in self.inflight if c.s < c.n_steps]
+            lane.inflight = [c for c in lane.inflight if c.s < c.n_steps]
             return True
         tile = self.tile
         if self.auto_tile:
@@ -387,27 +607,39 @@ class SolveServer:
         self.tiles_used[tile] = self.tiles_used.get(tile, 0) + 1
         npad = (n + 7) // 8 * 8
         rec["desc"][n:npad] = -1
-        self.ring_i = (k + 1) % self.n_ring
-        e_start = nat.Event()
-        e_start.record(self.ctx)
-        nat.call("lcm_memcpy_h2d_async", self.ctx.handle, C.c_void_p(self.task_dev[k].ptr),
-                 C.c_void_p(self.task_host[k].ptr), C.c_size_t(npad * TASK_DTYPE.itemsize))
-        nat.call("lcm_gs_launch_ex", self.ctx.handle, self.code, C.c_void_p(self.desc_buf.ptr),
-                 C.c_void_p(self.task_dev[k].ptr), npad, tile, C.c_size_t(lds), int(variant))
-        self.ring_events[k].record(self.ctx)
-        e_end = nat.Event()
-        e_end.record(self.ctx)
-        self._launch_events.append((e_start, e_end))
-        self.ring_used[k] = True
-        self.launches += 1
-        self.launch_tasks += n
+        lane.ring_i = (k + 1) % lane.n_ring
         done = []
         for c in picks:
             done += c.finished_at(c.s)
             c.s += 1
+        copies = []
+        if self.res_slab is not None:
+            for r in done:
+                copies += self._stage_result(r)
+        j = lane.tev_i
+        half = lane.TIMING // 2
+        if j % half == 0:
+            # entering a half of the ring: read back its pairs before reuse
+            # (recorded a half ring or more of launches ago, so finished)
+            lane.harvest(self._t0_event, j, j + half)
+        e_start, e_end = lane.tev[j]
+        lane.tev_live[j] = True
+        lane.tev_i = (j + 1) % lane.TIMING
+        nc = len(copies)
+        dst = (C.c_void_p * max(nc, 1))(*[d for d, _, _ in copies])
+        src = (C.c_void_p * max(nc, 1))(*[s_ for _, s_, _ in copies])
+        nby = (C.c_size_t * max(nc, 1))(*[b for _, _, b in copies])
+        # one foreign call: events, task upload, launch, result copies
+        nat.call("lcm_gs_serve_launch", lane.ctx.handle, self.code, C.c_void_p(self.desc_buf.ptr),
+                 C.c_void_p(lane.task_host[k].ptr), C.c_void_p(lane.task_dev[k].ptr), npad, tile,
+                 C.c_size_t(lds), int(variant), e_start.handle, lane.ring_events[k].handle, dst, src, nby, nc,
+                 e_end.handle)
+        lane.ring_used[k] = True
+        self.launches += 1
+        self.launch_tasks += n
         if done:
-            self.finishing.append((e_end, done))
-        self.inflight = [c for c in self.inflight if c.s < c.n_steps]
+            lane.finishing.append((e_end, done))
+        lane.inflight = [c for c in lane.inflight if c.s < c.n_steps]
         return True
 
     @classmethod
@@ -420,17 +652,25 @@ class SolveServer:
         return r[r["w0"] < n_w_of_desc[r["desc"]]]
 
     def _retire(self, block: bool = False):
-        """Complete the requests whose last step has run. Events were recorded
-        on one stream, so they complete in order: polling stops at the first
-        one still pending."""
+        """Complete the requests whose last step has run. Each lane's events
+        were recorded on its own stream, so they complete in order: polling a
+        lane stops at its first one still pending."""
         freed = False
-        while self.finishing:
-            ev, reqs = self.finishing[0]
+        for lane in self.lanes:
+            freed |= self._retire_lane(lane, block)
+        if freed:
+            with self._mem_cv:
+                self._mem_cv.notify_all()
+
+    def _retire_lane(self, lane: _Lane, block: bool) -> bool:
+        freed = False
+        while lane.finishing:
+            ev, reqs = lane.finishing[0]
             if block:
                 ev.synchronize()
             elif not ev.query():
                 break
-            self.finishing.pop(0)
+            lane.finishing.pop(0)
             t_done = time.perf_counter()
             for req in reqs:
                 req.t_done = t_done
@@ -445,13 +685,40 @@ class SolveServer:
             with self._ready_lock:
                 self.completed.extend(reqs)
             freed = True
-        if freed:
-            with self._mem_cv:
-                self._mem_cv.notify_all()
+        return freed
+
+    def _stage_result(self, req: Request) -> list:
+        """Pinned host slots for a finished request's period-0 V; returns the
+        (dst, src, bytes) copies that its last launch enqueues after the kernel."""
+        prog = req.prog
+        outs = prog.outs[0]
+        staged, copies = [], []
+        for key in [k for k in outs if k[0] == 0]:
+            o = outs[key]
+            n = int(np.prod(o.shape, dtype=np.int64)) if o.shape else 1
+            nb = n * self.dt.itemsize
+            sl = self.res_slab.alloc(nb)
+            if sl is None:  # no room: this request is copied synchronously when it retires
+                for _, s2, _, _ in staged:
+                    self.res_slab.release(*s2)
+                return []
+            staged.append((key, sl, n, o.shape))
+            copies.append((self.res_buf.ptr + sl[0], prog.base + o.v_off, nb))
+        req.staged = staged
+        return copies
 
     def _collect(self, req: Request):
         prog = req.prog
         if self.fetch_mode == "none":
+            return
+        if req.staged is not None:
+            res = {}
+            for (t, r), sl, n, shape in req.staged:
+                v = np.frombuffer(self.res_buf.array, dtype=self.dt, count=n, offset=sl[0]).copy()
+                res.setdefault(t, {})[r] = v.reshape(shape)
+                self.res_slab.release(*sl)
+            req.staged = None
+            req.result = res
             return
         outs = prog.outs[0]
         keys = sorted(outs) if self.fetch_mode in ("all", "keep") else [k for k in outs if k[0] == 0]
@@ -475,7 +742,9 @@ class SolveServer:
         t0 = time.perf_counter()
         self._admit()
         t1 = time.perf_counter()
-        issued = self._launch()
+        issued = False
+        for lane in self.lanes:
+            issued |= self._launch(lane)
         t2 = time.perf_counter()
         self._retire()
         t3 = time.perf_counter()
@@ -485,9 +754,23 @@ class SolveServer:
         return issued
 
     def gpu_busy_seconds(self) -> float:
-        """Sum of the fused launches' durations on the stream (task upload
-        included), from events; call after `drain`."""
-        return sum(a.elapsed_ms(b) for a, b in self._launch_events) / 1e3
+        """Time the device had at least one fused launch running (task upload
+        included): the union of the launches' intervals on all lanes, from
+        events; call after `drain`."""
+        iv = []
+        for lane in self.lanes:
+            lane.harvest(self._t0_event, 0, lane.TIMING)
+            iv += lane.intervals
+        iv.sort()
+        busy, end = 0.0, -1.0
+        for a, b in iv:
+            if a > end:
+                busy += b - a
+                end = b
+            elif b > end:
+                busy += b - end
+                end = b
+        return busy / 1e3
 
     def drain(self, timeout: float = 600.0):
         t_end = time.perf_counter() + timeout

@@ -242,3 +242,28 @@ def test_assembled_batch_mixes_problems_with_different_pointer_fields():
             has = ref != 0
             assert np.all(got[has] - ref[has] == ab.base + int(ab.out_offs[i]))
     assert tm_model.n_periods == 3
+
+
+def test_worker_result_spill_round_trip(tmp_path):
+    """A worker's bulky arrays written to a spill file and read back equal the
+    pickled ones (table image, descriptor records, per-step task rows), and
+    the file is removed."""
+    import cloudpickle
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd.serving import _lower_in_worker, _unspill
+
+    m = ex.retirement_model(7, n_wealth=300, n_consumption=300, income_nodes=5)
+    p = ex.retirement_params(7, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0})
+    blob = cloudpickle.dumps((m, p))
+    plain = _lower_in_worker(blob, "<f4", 256)
+    spilled = _lower_in_worker(blob, "<f4", 256, str(tmp_path / "req_"))
+    assert "image" not in spilled and len(list(tmp_path.iterdir())) == 1
+    image = _unspill(spilled)
+    assert list(tmp_path.iterdir()) == []
+    assert np.array_equal(image, plain["image"].view(np.uint8).ravel())
+    assert spilled["D0"].tobytes() == plain["D0"].tobytes()
+    assert len(spilled["tasks_host"]) == len(plain["tasks_host"])
+    for a, b in zip(spilled["tasks_host"], plain["tasks_host"]):
+        assert np.array_equal(a, b)
