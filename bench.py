@@ -385,8 +385,15 @@ def _serve_window(args, rate, seconds, seed):
     old_si = _sys.getswitchinterval()
     if args.serve_switch_ms > 0:
         _sys.setswitchinterval(args.serve_switch_ms * 1e-3)
-    server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=args.serve_threads,
-                         lower_procs=args.serve_procs, lanes=args.serve_lanes, queue_depth=args.serve_depth)
+    if args.serve_loop == "native":
+        from pylcm_amd.serving import NativeSolveServer
+
+        server = NativeSolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first",
+                                   lower_procs=args.serve_procs, lanes=args.serve_lanes,
+                                   queue_depth=args.serve_depth, placers=args.serve_placers)
+    else:
+        server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=args.serve_threads,
+                             lower_procs=args.serve_procs, lanes=args.serve_lanes, queue_depth=args.serve_depth)
     make = ex.serving_request_factory(seed=7)
     t0 = time.perf_counter()
     reqs = serve_poisson(server, make, rate=rate, duration=seconds, seed=seed)
@@ -402,24 +409,29 @@ def _serve_window(args, rate, seconds, seed):
            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
            "fused_launches": server.launches,
            "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
-           "host_loop_s": {k: round(v, 3) for k, v in server.timing.items()},
            "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
            "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())},
-           "lanes": len(server.lanes)}
+           "loop": args.serve_loop, "lanes": args.serve_lanes}
+    native = args.serve_loop == "native"
+    host = server.host_seconds() if native else dict(server.timing)
+    out["host_loop_s"] = {k: round(v, 3) for k, v in host.items()}
     # where the latency goes: lowering (worker processes), placement (device
     # memory wait + uploads), and the solve (admission to completion)
     stages = {"lower": [r.t_lowered - r.t_submit for r in ok],
               "place": [r.t_admit - r.t_lowered for r in ok],
-              "solve": [r.t_done - r.t_admit for r in ok]}
+              "solve": [(r.t_retired if native else r.t_done) - r.t_admit for r in ok]}
+    if native:  # completion in the library to the result in the client's hands
+        stages["collect"] = [r.t_done - r.t_retired for r in ok]
     out["stage_ms"] = {k: {"p50": round(1e3 * float(np.percentile(v, 50)), 2),
                            "p99": round(1e3 * float(np.percentile(v, 99)), 2)} for k, v in stages.items()}
     # capacities of the three resources a request passes through, in
     # requests/s: the lowering processes (their mean time per request in the
     # worker), the device (fused-launch busy time per request), and the
-    # serving loop's own host time (admit + launch + retire per request)
+    # serving loop's own host time (admit + launch + retire per request; for
+    # the native loop the library lanes' time assembling and issuing launches)
     wk = np.array([r.worker_s for r in ok if r.worker_s > 0])
     busy = out["gpu_busy_s"]
-    loop_s = sum(server.timing.values())
+    loop_s = host["launch"] if native else sum(host.values())
     out["capacity_req_per_s"] = {
         "lowering": round(args.serve_procs / float(wk.mean()), 1) if wk.size else None,
         "lowering_worker_ms_mean": round(1e3 * float(wk.mean()), 2) if wk.size else None,
@@ -466,7 +478,7 @@ def serving_leg(args):
     return {
         "workload": f"Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g}s; deterministic and AR(1)-income "
                     "retirement models, T U{20..60}, grids U{200..4000}, float32, period-0 V returned; "
-                    f"requests lowered in {args.serve_procs} worker processes",
+                    f"requests lowered in {args.serve_procs} worker processes, {args.serve_loop} serving loop",
         **win,
         "saturation": sat,
         "plain_serial": {"solves_per_s": round(plain_sps, 2),
@@ -551,13 +563,16 @@ def main():
     ap.add_argument("--sim-cpu-agents", type=int, default=1500)
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-sat-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-sat-rate", type=float, default=2000.0)
     ap.add_argument("--serve-sat-seconds", type=float, default=3.0)
     ap.add_argument("--serve-procs", type=int, default=12)
     ap.add_argument("--serve-plain-sample", type=int, default=40)
     ap.add_argument("--serve-tile", type=int, default=0, help="0: chosen per fused launch")
-    ap.add_argument("--serve-lanes", type=int, default=1, help="streams carrying the fused launches")
-    ap.add_argument("--serve-threads", type=int, default=4, help="threads placing lowered requests")
+    ap.add_argument("--serve-lanes", type=int, default=2, help="streams carrying the fused launches")
+    ap.add_argument("--serve-threads", type=int, default=4, help="threads placing lowered requests (python loop)")
+    ap.add_argument("--serve-loop", choices=("native", "python"), default="native",
+                    help="serving loop in library threads (native) or in this process (python)")
+    ap.add_argument("--serve-placers", type=int, default=2, help="library threads placing requests (native loop)")
     ap.add_argument("--serve-depth", type=int, default=2, help="fused launches queued per lane")
     ap.add_argument("--serve-switch-ms", type=float, default=0.0,
                     help="interpreter thread switch interval during serving (0: unchanged)")

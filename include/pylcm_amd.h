@@ -351,6 +351,109 @@ int lcm_gs_serve_launch(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs, const
 int lcm_event_elapsed_batch(void* ref, void* const* evs, int n, float* out_ms);
 
 /* ------------------------------------------------------------------------- */
+/* Native solve server (continuous batching of grid-search solves)            */
+/* ------------------------------------------------------------------------- */
+/*
+ * None of this has a counterpart in the reference, which solves one model at
+ * a time from a host loop (src/_lcm/solution/backward_induction.py:155-275)
+ * and has no server. The server's library threads place requests in device
+ * memory, admit them to lanes (one HIP stream each), issue one fused
+ * grid-search launch per lane holding the next backward step of every
+ * request in flight there, and retire requests whose last step has run; the
+ * host process only writes request files and collects results.
+ *
+ * Request file (little-endian): an LcmReqHeader; int64 arrays step_count,
+ * step_g, step_k, step_variant (n_steps each: tasks of each backward step,
+ * its largest next grid and constraint count for lcm_gs_lds_bytes, its
+ * kernel variant), fetch_off, fetch_bytes (n_fetch each: outputs copied
+ * back, as byte offsets from the request's base), uint64 ptr_mask (n_desc:
+ * bit i set when 8-byte word i of the descriptor record is a pointer
+ * relative to the base); then the n_desc descriptor records for base 0, the
+ * n_task task rows {desc, sd, w0} as int32 (desc local to the request, steps
+ * in order, 256-state tasks), and the table image (image_bytes, placed at
+ * base + out_start). A request occupies total_bytes of device memory.
+ */
+#define LCM_REQ_MAGIC 0x31305145524d434cULL /* "LCMREQ01" */
+#define LCM_REQ_TILE 256                    /* states per task row */
+#define LCM_DESC_PTR_WORDS 36               /* leading pointer words of LcmGsDesc */
+
+typedef struct LcmReqHeader {
+  uint64_t magic;
+  int64_t dtype, n_steps, n_desc, n_task, n_fetch;
+  int64_t image_bytes, out_start, total_bytes, tile;
+  int64_t reserved[6];
+} LcmReqHeader;
+
+/* Parse and check a request file without a device: info[0..8) = n_steps,
+ * n_desc, n_task, n_fetch, image_bytes, total_bytes, sum of fetch_bytes,
+ * pointer words flagged in ptr_mask; info[8] = largest step variant,
+ * info[9] = largest lcm_gs_lds_bytes of a step. Returns LCM_OK or
+ * LCM_E_ARG with the reason in lcm_last_error(). */
+int lcm_req_inspect(const char* path, int64_t* info, int n_info);
+
+/* The tile the autotuner picks for a launch of `units` 256-state tasks whose
+ * states scan at most n_actions actions: that of the nearest entry (squared
+ * distance in log2 units and log2 actions; the first of equals) of the
+ * measured table (units[i], actions[i]) -> tiles[i]; with no entries 64 up
+ * to 256 units, 128 up to 2048, else 256. Host only. */
+int lcm_tile_choose(const int64_t* units, const int64_t* actions, const int32_t* tiles, int n, int64_t u,
+                    int64_t a);
+
+/* The `tile`-state tasks (tile 64 or 128) of n 256-state tasks: each task's
+ * parts w0 + j * tile (j < 256 / tile) that start below its descriptor's
+ * n_w (n_w_of_desc indexed by task desc), in order. Returns the count
+ * written to out (room for n * 256 / tile). Host only. */
+int64_t lcm_split_tasks(const LcmGsTask* in, int64_t n, const int32_t* n_w_of_desc, int tile, LcmGsTask* out);
+
+/* The server's clock: seconds of CLOCK_MONOTONIC (the clock of the times
+ * lcm_serve_fetch reports). */
+double lcm_serve_clock(void);
+
+typedef struct lcm_serve lcm_serve;
+
+/* A server on the current device computing in `dtype`. budget_bytes of
+ * device memory hold placed requests; max_desc descriptor slots; max_tasks
+ * task records per fused launch; max_inflight requests placed at once;
+ * result_bytes of pinned host memory hold fetched outputs until collected;
+ * `lanes` streams, each with queue_depth task buffers (launches queued
+ * ahead); `placers` placement threads. fixed_tile 64/128/256, or 0 to pick
+ * each launch's tile with lcm_tile_choose over the n_table entries. */
+int lcm_serve_create(lcm_serve** out, int dtype, size_t budget_bytes, int max_desc, int max_tasks,
+                     int max_inflight, size_t result_bytes, int lanes, int queue_depth, int placers,
+                     int fixed_tile, const int64_t* table_units, const int64_t* table_actions,
+                     const int32_t* table_tiles, int n_table);
+
+/* Queue request `rid` (any id unique among those not yet fetched) from a
+ * request file; the server removes the file once read. */
+int lcm_serve_submit(lcm_serve* s, int64_t rid, const char* path);
+
+/* Wait up to timeout_ms for completed requests; writes up to max_n of them
+ * (id, status: LCM_OK or an error code) and returns how many. */
+int lcm_serve_wait(lcm_serve* s, int64_t* rids, int32_t* status, int max_n, int timeout_ms);
+
+/* Take a completed request: copies its fetched outputs (in file order,
+ * packed) to dst when its status is LCM_OK and nbytes covers them; times[3]
+ * = submitted, placed, done (lcm_serve_clock); err receives the message of
+ * a failed request. Returns the request's status (LCM_E_ARG for an unknown
+ * id). */
+int lcm_serve_fetch(lcm_serve* s, int64_t rid, void* dst, size_t nbytes, double* times, char* err,
+                    size_t err_len);
+
+/* Counters: out[0..12) = launches, task records launched, launches at tile
+ * 64, 128, 256, nanoseconds the lanes spent assembling and issuing launches,
+ * nanoseconds the placers spent placing, requests placed, requests retired,
+ * device bytes in use, device bytes of the budget, the most ever in use. */
+int lcm_serve_stats(const lcm_serve* s, int64_t* out, int n);
+
+/* Milliseconds the device had at least one fused launch running (the union
+ * over lanes of each launch's task upload and kernel, from events). */
+int lcm_serve_busy_ms(lcm_serve* s, double* ms);
+
+/* Stop the threads (requests not completed are dropped; their request files
+ * removed), wait for the lanes' streams and free everything. */
+int lcm_serve_destroy(lcm_serve* s);
+
+/* ------------------------------------------------------------------------- */
 /* DC-EGM with the FUES upper envelope (all arithmetic FP64)                  */
 /* ------------------------------------------------------------------------- */
 

@@ -24,6 +24,9 @@ MAX_CONSTR = 4
 MAX_SCALAR = 4
 MAX_TASTE_D = 8
 ACT_CHUNK = 256
+REQ_MAGIC = 0x31305145524D434C  # "LCMREQ01", the request files of the native solve server
+REQ_TILE = 256
+DESC_PTR_WORDS = 36
 
 vp = C.c_void_p
 
@@ -193,6 +196,18 @@ _SIGS = {
     "lcm_egm_lds_bytes": ([C.c_int], C.c_size_t),
     "lcm_abi_sizes_egm": ([C.POINTER(C.c_size_t)] * 3, C.c_int),
     "lcm_tape_eval": ([vp, vp, vp, vp, vp, vp, C.c_int], C.c_int),
+    "lcm_req_inspect": ([C.c_char_p, C.POINTER(C.c_int64), C.c_int], C.c_int),
+    "lcm_tile_choose": ([vp, vp, vp, C.c_int, C.c_int64, C.c_int64], C.c_int),
+    "lcm_split_tasks": ([vp, C.c_int64, vp, C.c_int, vp], C.c_int64),
+    "lcm_serve_clock": ([], C.c_double),
+    "lcm_serve_create": ([C.POINTER(vp), C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_int,
+                          C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int], C.c_int),
+    "lcm_serve_submit": ([vp, C.c_int64, C.c_char_p], C.c_int),
+    "lcm_serve_wait": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int, C.c_int], C.c_int),
+    "lcm_serve_fetch": ([vp, C.c_int64, vp, C.c_size_t, C.POINTER(C.c_double), C.c_char_p, C.c_size_t], C.c_int),
+    "lcm_serve_stats": ([vp, C.POINTER(C.c_int64), C.c_int], C.c_int),
+    "lcm_serve_busy_ms": ([vp, C.POINTER(C.c_double)], C.c_int),
+    "lcm_serve_destroy": ([vp], C.c_int),
     "lcm_philox": ([vp, vp, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32], C.c_int),
 }
 
@@ -247,6 +262,11 @@ def lib():
 def call(name: str, *args):
     L = lib()
     _check(getattr(L, name)(*args), name)
+
+
+def host_error() -> str:
+    """The library's last error message (host-only entry points)."""
+    return load_library().lcm_last_error().decode(errors="replace")
 
 
 def device_info():

@@ -66,6 +66,71 @@ def _lower_in_worker(blob: bytes, dtype: str, tile: int, spill: str | None = Non
     return data
 
 
+def write_request_file(path: str, data: dict, dtype, fetch: str = "first") -> list:
+    """Write one packed request (`pack_problem` output with 256-state tasks)
+    as a request file of the native server (layout in `lcm_serve_submit`'s
+    header: `LcmReqHeader`, per-step arrays, fetched outputs, pointer masks,
+    descriptor records at base 0, task rows, table image). Returns the
+    outputs it fetches, in file order: [(period, regime, shape, bytes)]."""
+    dt = np.dtype(dtype)
+    D0, masks = data["D0"], data["masks"]
+    n = len(D0)
+    ptr = np.zeros(n, dtype=np.uint64)
+    for f, m in masks.items():
+        w0 = DESC_DTYPE.fields[f][1] // 8
+        m2 = np.asarray(m, dtype=bool).reshape(n, -1)
+        for j in range(m2.shape[1]):
+            ptr |= m2[:, j].astype(np.uint64) << np.uint64(w0 + j)
+    outs = data["outs"][0]
+    if fetch == "first":
+        keys = [k for k in outs if k[0] == 0]
+    elif fetch == "all":
+        keys = sorted(outs)
+    elif fetch == "none":
+        keys = []
+    else:
+        raise ValueError(f"fetch must be 'first', 'all' or 'none', not {fetch!r}")
+    fetched, f_off, f_bytes = [], [], []
+    for t, r in keys:
+        o = outs[(t, r)]
+        nb = (int(np.prod(o.shape, dtype=np.int64)) if o.shape else 1) * dt.itemsize
+        fetched.append((t, r, tuple(o.shape), nb))
+        f_off.append(o.v_off)
+        f_bytes.append(nb)
+    n_steps = int(data["n_steps"])
+    cat = np.ascontiguousarray(data["tasks_cat"], dtype=np.int32).reshape(-1, 3)
+    img = np.ascontiguousarray(data["image"]).view(np.uint8)
+    code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
+    header = np.array([nat.REQ_MAGIC, code, n_steps, n, len(cat), len(keys), img.nbytes, data["out_start"],
+                       data["total_bytes"], nat.REQ_TILE] + [0] * 6, dtype=np.int64)
+    per_step = np.concatenate([np.asarray(data[k], dtype=np.int64).reshape(n_steps)
+                               for k in ("task_counts", "lds_g", "lds_k", "wdep_by_step")]
+                              + [np.asarray(f_off, dtype=np.int64), np.asarray(f_bytes, dtype=np.int64)])
+    with open(path, "wb") as f:
+        for part in (header, per_step, ptr, D0, cat, img):
+            f.write(memoryview(np.ascontiguousarray(part)).cast("B"))
+    return fetched
+
+
+def _lower_to_request_file(blob, dtype: str, spill: str, fetch: str) -> dict:
+    """Lower one request in a worker and write it as a native request file
+    (a path of the `spill` prefix); returns the path and the fetched outputs.
+    `blob` is the cloudpickled (model, params), or the pair itself."""
+    from pylcm_amd._batch import pack_problem
+
+    t0 = time.perf_counter()
+    if isinstance(blob, bytes):
+        import cloudpickle
+
+        model, params = cloudpickle.loads(blob)
+    else:
+        model, params = blob
+    data = pack_problem(model, params, dtype, nat.REQ_TILE)
+    path = f"{spill}{os.getpid()}_{threading.get_ident()}_{time.perf_counter_ns()}"
+    fetched = write_request_file(path, data, dtype, fetch)
+    return {"path": path, "fetched": fetched, "worker_s": time.perf_counter() - t0}
+
+
 def _unspill(data: dict) -> np.ndarray:
     """Read back what `_lower_in_worker` spilled: fills data's D0 and
     tasks_host (copies) and returns the table image (a copy in host memory);
@@ -102,9 +167,12 @@ def _pool_worker_main(conn, spill):
             return
         if msg is None:
             return
-        rid, dtype, tile, blob = msg
+        rid, dtype, tile, blob, fetch = msg
         try:
-            out = (rid, _lower_in_worker(blob, dtype, tile, spill))
+            if fetch is None:
+                out = (rid, _lower_in_worker(blob, dtype, tile, spill))
+            else:  # a request file for the native server
+                out = (rid, _lower_to_request_file(blob, dtype, spill, fetch))
         except BaseException as e:  # surfaced to the client of this request
             out = (rid, e)
         conn.send(out)
@@ -140,11 +208,13 @@ class _LoweringPool:
         self._rx = threading.Thread(target=self._receive, daemon=True)
         self._rx.start()
 
-    def submit(self, rid: int, dtype: str, tile: int, blob: bytes):
+    def submit(self, rid: int, dtype: str, tile: int, blob: bytes, fetch: str | None = None):
+        """Queue a request; `fetch` set: the worker writes a native request
+        file fetching those outputs instead of returning the packed arrays."""
         with self._lock:
             k = min(range(len(self.conns)), key=lambda j: len(self.outstanding[j]))
             self.outstanding[k].add(rid)
-        self.conns[k].send((rid, dtype, tile, blob))
+        self.conns[k].send((rid, dtype, tile, blob, fetch))
 
     def _receive(self):
         from multiprocessing.connection import wait
@@ -240,7 +310,9 @@ class Request:
     staged: list | None = None  # period-0 V copied to pinned host memory by the stream
     n_act_steps: list | None = None  # most actions per state of each step's tasks
     t_admit: float = 0.0
+    t_retired: float = 0.0  # native server: its last launch retired (before collection)
     t_done: float = 0.0
+    fetched: list | None = None  # native server: [(period, regime, shape, bytes)] in file order
     slab: tuple | None = None
     desc: tuple | None = None
     tasks: list | None = None
@@ -783,6 +855,258 @@ class SolveServer:
         self._retire(block=True)
 
 
+def _spill_prefix() -> str:
+    """A fresh path prefix for request files (memory-backed /dev/shm when present)."""
+    base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
+    return os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
+
+
+class NativeSolveServer:
+    """Continuous batching of grid-search solves with the serving loop in the
+    library (`lcm_serve_*`): its threads place requests in device memory,
+    admit them to lanes (one HIP stream each), issue one fused launch per lane
+    holding the next backward step of every request in flight there, copy
+    each finished request's outputs to pinned host memory behind its last
+    launch, and retire launches as their end events fire. This process only
+    lowers requests into request files (in `lower_procs` worker processes, or
+    in `lower_threads` threads when `lower_procs` is 0) and collects results,
+    so the launch cadence never waits on the interpreter.
+
+    The same kernel, descriptors and per-launch tile choice as `SolveServer`
+    solve each request (the tile does not change results), so a request's
+    values equal its plain solve bit for bit. `fetch`: "first" (period-0 V),
+    "all" (every period's V) or "none". `tile` 0 picks each launch's tile from
+    `tile_table` (default: the table measured on this GPU model).
+    As with any spawned pool, a main script using `lower_procs` must guard its
+    entry point with `if __name__ == "__main__":`.
+    """
+
+    loop_in_library = True
+
+    def __init__(self, *, dtype=np.float32, tile: int = 0, budget_bytes: int | None = None,
+                 free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
+                 max_tasks: int = 1 << 20, fetch: str = "first", lower_procs: int = 4, lower_threads: int = 2,
+                 tile_table: dict | None = None, lanes: int = 1, queue_depth: int = 2, placers: int = 2,
+                 result_pinned_bytes: int = 64 << 20):
+        if fetch not in ("first", "all", "none"):
+            raise ValueError(f"fetch must be 'first', 'all' or 'none', not {fetch!r}")
+        self.dt = np.dtype(dtype)
+        self.code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
+        self.fetch_mode = fetch
+        self.n_lanes = max(1, int(lanes))
+        _, free, _ = nat.device_info()
+        budget = int(free * free_fraction)
+        if budget_bytes is not None:
+            budget = min(budget, int(budget_bytes))
+        table = tile_table if tile_table is not None else autotune.load_table()
+        dts = "f64" if self.dt == np.float64 else "f32"
+        ent = [e for e in table.get("entries", []) if e["dtype"] == dts]
+        self._t_units = np.array([e["units"] for e in ent], dtype=np.int64)
+        self._t_acts = np.array([e["n_actions"] for e in ent], dtype=np.int64)
+        self._t_tiles = np.array([e["tile"] for e in ent], dtype=np.int32)
+        self._L = nat.lib()
+        self._h = C.c_void_p()
+        nat.call("lcm_serve_create", C.byref(self._h), self.code, C.c_size_t(budget), int(max_desc),
+                 int(max_tasks), int(max_inflight), C.c_size_t(int(result_pinned_bytes)), self.n_lanes,
+                 int(queue_depth), int(placers), int(tile), self._t_units.ctypes.data, self._t_acts.ctypes.data,
+                 self._t_tiles.ctypes.data, len(ent))
+        self.budget = budget
+        self._spill = _spill_prefix()
+        self._lock = threading.Lock()
+        self._by_rid: dict[int, Request] = {}
+        self.completed: list[Request] = []
+        self._next_id = 0
+        self.timing = {"collect": 0.0}
+        self._closing = False
+        self._pool = None
+        self._lowq: queue.Queue | None = None
+        self._workers = []
+        if lower_procs > 0:
+            self._pool = _LoweringPool(int(lower_procs), self._spill, self._on_lowered)
+        else:
+            self._lowq = queue.Queue()
+            self._workers = [threading.Thread(target=self._lower_worker, daemon=True)
+                             for _ in range(max(1, int(lower_threads)))]
+            for w in self._workers:
+                w.start()
+        self._collector = threading.Thread(target=self._collect_main, daemon=True)
+        self._collector.start()
+
+    # -- client side --------------------------------------------------------------
+    def submit(self, model, params, t_submit: float | None = None) -> Request:
+        req = Request(self._next_id, model, params, t_submit if t_submit is not None else time.perf_counter())
+        self._next_id += 1
+        with self._lock:
+            self._by_rid[req.rid] = req
+        if self._pool is not None:
+            import cloudpickle
+
+            self._pool.submit(req.rid, self.dt.str, nat.REQ_TILE, cloudpickle.dumps((model, params)),
+                              self.fetch_mode)
+        else:
+            self._lowq.put(req)
+        return req
+
+    def pending(self) -> int:
+        """Requests submitted and not yet completed."""
+        with self._lock:
+            return self._next_id - len(self.completed)
+
+    def step(self) -> bool:
+        """Nothing to do on the caller's thread: the library runs the loop."""
+        return False
+
+    @property
+    def finishing(self) -> list:
+        return []
+
+    def _retire(self, block: bool = False):
+        pass
+
+    def drain(self, timeout: float = 600.0):
+        t_end = time.perf_counter() + timeout
+        while self.pending() and time.perf_counter() < t_end:
+            time.sleep(0.001)
+
+    # -- lowering -----------------------------------------------------------------
+    def _lower_worker(self):
+        while True:
+            req = self._lowq.get()
+            if req is None:
+                return
+            try:
+                out = _lower_to_request_file((req.model, req.params), self.dt.str, self._spill, self.fetch_mode)
+            except BaseException as e:  # surfaced to the client
+                out = e
+            self._on_lowered(req.rid, out, -1)
+
+    def _on_lowered(self, rid, out, worker: int):
+        """Hand a lowered request's file to the library (or fail the request)."""
+        with self._lock:
+            req = self._by_rid.get(rid)
+        if req is None:
+            return
+        req.t_lowered = time.perf_counter()
+        if not isinstance(out, BaseException):
+            req.worker_s = out["worker_s"]
+            req.fetched = out["fetched"]
+            if self._L.lcm_serve_submit(self._h, rid, out["path"].encode()) == 0:
+                return
+            out = nat.NativeKernelError(f"lcm_serve_submit failed: {nat.host_error()}")
+        with self._lock:
+            self._by_rid.pop(rid, None)
+        req.error = out
+        req.t_done = time.perf_counter()
+        req.done.set()
+        with self._lock:
+            self.completed.append(req)
+
+    # -- collection ---------------------------------------------------------------
+    def _collect_main(self):
+        L, h = self._L, self._h
+        cap = 256
+        rids = (C.c_int64 * cap)()
+        st = (C.c_int32 * cap)()
+        times = (C.c_double * 3)()
+        err = C.create_string_buffer(1024)
+        while True:
+            n = L.lcm_serve_wait(h, rids, st, cap, 50)
+            if n <= 0:
+                if self._closing:
+                    return
+                continue
+            t0 = time.perf_counter()
+            done = []
+            for i in range(n):
+                rid = int(rids[i])
+                with self._lock:
+                    req = self._by_rid.pop(rid, None)
+                fetched = req.fetched if req is not None else []
+                total = sum(f[3] for f in fetched)
+                buf = np.empty(total, dtype=np.uint8)
+                rc = L.lcm_serve_fetch(h, rid, buf.ctypes.data if total else None, total, times, err, len(err))
+                if req is None:
+                    continue
+                req.t_admit, req.t_retired = times[1], times[2]
+                if rc == 0:
+                    res, o = {}, 0
+                    if self.fetch_mode != "none":
+                        for t, r, shape, nb in fetched:
+                            res.setdefault(t, {})[r] = buf[o: o + nb].view(self.dt).reshape(shape)
+                            o += nb
+                    req.result = res
+                else:
+                    req.error = nat.NativeKernelError(err.value.decode(errors="replace"), status=rc)
+                req.t_done = time.perf_counter()
+                req.done.set()
+                done.append(req)
+            with self._lock:
+                self.completed.extend(done)
+            self.timing["collect"] += time.perf_counter() - t0
+
+    # -- statistics ---------------------------------------------------------------
+    def stats(self) -> dict:
+        """The library's counters (after `close`, their final values)."""
+        if self._h is None:
+            return dict(self._final["stats"])
+        v = (C.c_int64 * 12)()
+        nat.call("lcm_serve_stats", self._h, v, 12)
+        keys = ("launches", "tasks", "tile64", "tile128", "tile256", "loop_ns", "place_ns", "placed", "retired",
+                "bytes_used", "bytes_budget", "bytes_peak")
+        return dict(zip(keys, (int(x) for x in v)))
+
+    @property
+    def launches(self) -> int:
+        return self.stats()["launches"]
+
+    @property
+    def launch_tasks(self) -> int:
+        return self.stats()["tasks"]
+
+    @property
+    def tiles_used(self) -> dict:
+        s = self.stats()
+        return {t: s[f"tile{t}"] for t in (64, 128, 256) if s[f"tile{t}"]}
+
+    def host_seconds(self) -> dict:
+        """Seconds the library's lanes spent assembling and issuing launches,
+        its placers spent placing, and this process's collector spent
+        collecting."""
+        s = self.stats()
+        return {"launch": s["loop_ns"] / 1e9, "place": s["place_ns"] / 1e9, "collect": self.timing["collect"]}
+
+    def gpu_busy_seconds(self) -> float:
+        """Time the device had at least one fused launch running (task upload
+        included), the union over lanes; call after `drain`."""
+        if self._h is None:
+            return self._final["busy_s"]
+        ms = C.c_double()
+        nat.call("lcm_serve_busy_ms", self._h, C.byref(ms))
+        return ms.value / 1e3
+
+    def close(self):
+        if self._h is None:
+            return
+        self._closing = True
+        for _ in self._workers:
+            self._lowq.put(None)
+        if self._pool is not None:
+            self._pool.close()
+        self._collector.join(timeout=60)
+        if self._collector.is_alive():  # never free the server under a waiting collector
+            raise RuntimeError("NativeSolveServer.close: the collector thread did not stop")
+        self._final = {"stats": self.stats(), "busy_s": self.gpu_busy_seconds()}
+        self._L.lcm_serve_destroy(self._h)
+        self._h = None
+        import glob
+
+        for path in glob.glob(self._spill + "*"):
+            try:
+                os.unlink(path)
+            except OSError:
+                pass
+
+
 def serve_poisson(server: SolveServer, make_request, rate: float, duration: float, seed: int = 0):
     """Submit requests with exponential inter-arrival times (rate per second)
     for `duration` seconds while running the serving loop; then drain.
@@ -799,7 +1123,10 @@ def serve_poisson(server: SolveServer, make_request, rate: float, duration: floa
             next_t += rng.exponential(1.0 / rate)
         if next_t - t0 >= duration and now - t0 >= duration:
             break
-        if not server.step():
+        if getattr(server, "loop_in_library", False):
+            # nothing to run here: sleep until the next arrival
+            time.sleep(min(max(next_t - time.perf_counter(), 0.0), 0.005))
+        elif not server.step():
             if server.finishing:
                 server._retire()
             time.sleep(0.0002)

@@ -119,7 +119,7 @@ def test_library_exports_every_declared_symbol():
     from pylcm_amd import _native
 
     header = (ROOT / "include" / "pylcm_amd.h").read_text()
-    names = set(re.findall(r"^(?:int|const char\*|size_t)\s+(lcm_\w+)\(", header, re.M))
+    names = set(re.findall(r"^(?:int|int64_t|double|const char\*|size_t)\s+(lcm_\w+)\(", header, re.M))
     assert names, "no declarations found"
     lib = _native.load_library()
     for n in sorted(names):
