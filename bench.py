@@ -390,7 +390,8 @@ def _serve_window(args, rate, seconds, seed):
 
         server = NativeSolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first",
                                    lower_procs=args.serve_procs, lanes=args.serve_lanes,
-                                   queue_depth=args.serve_depth, placers=args.serve_placers)
+                                   queue_depth=args.serve_depth, placers=args.serve_placers,
+                                   order=args.serve_order)
     else:
         server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=args.serve_threads,
                              lower_procs=args.serve_procs, lanes=args.serve_lanes, queue_depth=args.serve_depth)
@@ -411,7 +412,8 @@ def _serve_window(args, rate, seconds, seed):
            "gpu_busy_s": round(server.gpu_busy_seconds(), 3),
            "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
            "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())},
-           "loop": args.serve_loop, "lanes": args.serve_lanes}
+           "loop": args.serve_loop, "lanes": args.serve_lanes,
+           "task_order": args.serve_order if args.serve_loop == "native" else "admission"}
     native = args.serve_loop == "native"
     host = server.host_seconds() if native else dict(server.timing)
     out["host_loop_s"] = {k: round(v, 3) for k, v in host.items()}
@@ -573,6 +575,8 @@ def main():
     ap.add_argument("--serve-loop", choices=("native", "python"), default="native",
                     help="serving loop in library threads (native) or in this process (python)")
     ap.add_argument("--serve-placers", type=int, default=2, help="library threads placing requests (native loop)")
+    ap.add_argument("--serve-order", choices=("longest", "admission"), default="longest",
+                    help="order of a fused launch's tasks (native loop)")
     ap.add_argument("--serve-depth", type=int, default=2, help="fused launches queued per lane")
     ap.add_argument("--serve-switch-ms", type=float, default=0.0,
                     help="interpreter thread switch interval during serving (0: unchanged)")

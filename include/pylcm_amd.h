@@ -417,11 +417,15 @@ typedef struct lcm_serve lcm_serve;
  * result_bytes of pinned host memory hold fetched outputs until collected;
  * `lanes` streams, each with queue_depth task buffers (launches queued
  * ahead); `placers` placement threads. fixed_tile 64/128/256, or 0 to pick
- * each launch's tile with lcm_tile_choose over the n_table entries. */
+ * each launch's tile with lcm_tile_choose over the n_table entries. flags:
+ * LCM_SERVE_LONGEST_FIRST orders each launch's tasks by estimated work
+ * (states x actions per state), longest first on every XCD; otherwise tasks
+ * run in admission order. Neither changes a result. */
+#define LCM_SERVE_LONGEST_FIRST 1
 int lcm_serve_create(lcm_serve** out, int dtype, size_t budget_bytes, int max_desc, int max_tasks,
                      int max_inflight, size_t result_bytes, int lanes, int queue_depth, int placers,
                      int fixed_tile, const int64_t* table_units, const int64_t* table_actions,
-                     const int32_t* table_tiles, int n_table);
+                     const int32_t* table_tiles, int n_table, int flags);
 
 /* Queue request `rid` (any id unique among those not yet fetched) from a
  * request file; the server removes the file once read. */

@@ -27,6 +27,7 @@ ACT_CHUNK = 256
 REQ_MAGIC = 0x31305145524D434C  # "LCMREQ01", the request files of the native solve server
 REQ_TILE = 256
 DESC_PTR_WORDS = 36
+SERVE_LONGEST_FIRST = 1
 
 vp = C.c_void_p
 
@@ -201,7 +202,7 @@ _SIGS = {
     "lcm_split_tasks": ([vp, C.c_int64, vp, C.c_int, vp], C.c_int64),
     "lcm_serve_clock": ([], C.c_double),
     "lcm_serve_create": ([C.POINTER(vp), C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_size_t, C.c_int,
-                          C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int], C.c_int),
+                          C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int], C.c_int),
     "lcm_serve_submit": ([vp, C.c_int64, C.c_char_p], C.c_int),
     "lcm_serve_wait": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int, C.c_int], C.c_int),
     "lcm_serve_fetch": ([vp, C.c_int64, vp, C.c_size_t, C.POINTER(C.c_double), C.c_char_p, C.c_size_t], C.c_int),

@@ -266,6 +266,8 @@ struct Lane {
   int load = 0;  // requests assigned and not retired (server mutex)
   std::thread th;
   // reused per launch
+  std::vector<std::pair<int64_t, int64_t>> keys;
+  std::vector<LcmGsTask> order_tmp;
   std::vector<void*> dst;
   std::vector<const void*> src;
   std::vector<size_t> bytes;
@@ -278,7 +280,7 @@ using namespace lcm;
 
 struct lcm_serve {
   int device = 0, dtype = 0;
-  int max_tasks = 0, max_inflight = 0, fixed_tile = 0;
+  int max_tasks = 0, max_inflight = 0, fixed_tile = 0, flags = 0;
   std::vector<int64_t> t_units, t_acts;
   std::vector<int32_t> t_tiles;
   char* slab_dev = nullptr;
@@ -286,6 +288,7 @@ struct lcm_serve {
   char* res_host = nullptr;
   Slab slab, desc_slab, res_slab;
   std::vector<int32_t> n_w_of_desc;  // by global descriptor index
+  std::vector<int64_t> act_of_desc;  // actions per state, by global descriptor index
   std::vector<Lane*> lanes;
   std::vector<std::thread> placers;
   hipEvent_t t0 = nullptr;
@@ -319,7 +322,32 @@ struct lcm_serve {
   void launch(Lane* L);
   bool retire_ready(Lane* L, bool block);
   hipEvent_t take_event(Lane* L);
+  void order_longest_first(Lane* L, LcmGsTask* h, int64_t n, int64_t npad, int tile);
 };
+
+// Reorder a launch's n tasks (npad with padding) so that the kernel's
+// workgroups start the longest ones first on every XCD: rank the tasks by
+// estimated work (states x actions per state, then the higher w0, whose
+// states usually have more feasible actions), and deal the ranks round-robin
+// over the eight contiguous runs the kernel hands to the eight XCDs (rank i
+// to run i % 8, position i / 8), so each run is longest-first and the runs
+// carry equal shares. Results do not depend on the order of tasks.
+void lcm_serve::order_longest_first(Lane* L, LcmGsTask* h, int64_t n, int64_t npad, int tile) {
+  auto& keys = L->keys;
+  keys.resize(size_t(n));
+  for (int64_t i = 0; i < n; ++i) {
+    const LcmGsTask& t = h[i];
+    const int64_t states = std::min<int64_t>(tile, int64_t(n_w_of_desc[size_t(t.desc)]) - t.w0);
+    keys[size_t(i)] = {-(states * act_of_desc[size_t(t.desc)] * 65536 + t.w0), i};
+  }
+  std::sort(keys.begin(), keys.end());
+  L->order_tmp.assign(h, h + npad);
+  const int64_t per = npad / 8;
+  for (int64_t r = 0; r < npad; ++r) {
+    const LcmGsTask& t = r < n ? L->order_tmp[size_t(keys[size_t(r)].second)] : L->order_tmp[size_t(r)];
+    h[(r % 8) * per + r / 8] = t;
+  }
+}
 
 hipEvent_t lcm_serve::take_event(Lane* L) {
   if (!L->ev_pool.empty()) {
@@ -431,6 +459,7 @@ int lcm_serve::place(Req* r, char*& stage, size_t& stage_bytes, hipStream_t stre
       w[__builtin_ctzll(mask)] += reinterpret_cast<uint64_t>(base);
     n_w_of_desc[size_t(desc_base + i)] = D[i].n_w;
     act[size_t(i)] = int64_t(D[i].n_d) * D[i].n_c;
+    act_of_desc[size_t(desc_base + i)] = act[size_t(i)];
   }
   hipError_t e = hipSuccess;
   if (h.image_bytes)
@@ -623,6 +652,7 @@ void lcm_serve::launch(Lane* L) {
   }
   const int64_t npad = (n + 7) / 8 * 8;
   for (int64_t j = n; j < npad; ++j) h[j] = LcmGsTask{-1, 0, 0, 0};
+  if ((flags & LCM_SERVE_LONGEST_FIRST) && n > 1) order_longest_first(L, h, n, npad, tile);
   // requests this launch completes: their outputs follow the kernel
   Launch rec;
   L->dst.clear();
@@ -778,8 +808,9 @@ int lcm_req_inspect(const char* path, int64_t* info, int n_info) {
 int lcm_serve_create(lcm_serve** out, int dtype, size_t budget_bytes, int max_desc, int max_tasks,
                      int max_inflight, size_t result_bytes, int lanes, int queue_depth, int placers,
                      int fixed_tile, const int64_t* table_units, const int64_t* table_actions,
-                     const int32_t* table_tiles, int n_table) {
+                     const int32_t* table_tiles, int n_table, int flags) {
   if (!out) return fail(LCM_E_ARG, "lcm_serve_create: null out");
+  if (flags & ~LCM_SERVE_LONGEST_FIRST) return fail(LCM_E_ARG, "lcm_serve_create: unknown flags");
   *out = nullptr;
   if (dtype != LCM_DTYPE_F32 && dtype != LCM_DTYPE_F64) return fail(LCM_E_ARG, "lcm_serve_create: dtype");
   if (max_desc <= 0 || max_tasks < 64 || max_inflight <= 0 || lanes <= 0 || lanes > 16 || queue_depth <= 0 ||
@@ -797,6 +828,7 @@ int lcm_serve_create(lcm_serve** out, int dtype, size_t budget_bytes, int max_de
   s->max_tasks = max_tasks / 8 * 8;
   s->max_inflight = max_inflight;
   s->fixed_tile = fixed_tile;
+  s->flags = flags;
   s->t_units.assign(table_units, table_units + n_table);
   s->t_acts.assign(table_actions, table_actions + n_table);
   s->t_tiles.assign(table_tiles, table_tiles + n_table);
@@ -810,6 +842,7 @@ int lcm_serve_create(lcm_serve** out, int dtype, size_t budget_bytes, int max_de
   s->desc_slab.init(size_t(max_desc) * sizeof(LcmGsDesc), sizeof(LcmGsDesc));
   s->res_slab.init(std::max<size_t>(result_bytes, 256), 256);
   s->n_w_of_desc.assign(size_t(max_desc), 0);
+  s->act_of_desc.assign(size_t(max_desc), 0);
   if ((e = hipMalloc(reinterpret_cast<void**>(&s->slab_dev), s->slab.size())) != hipSuccess)
     return cleanup(check_hip(e, "hipMalloc (server memory)"));
   if ((e = hipMalloc(reinterpret_cast<void**>(&s->desc_dev), s->desc_slab.size())) != hipSuccess)

@@ -877,6 +877,9 @@ class NativeSolveServer:
     values equal its plain solve bit for bit. `fetch`: "first" (period-0 V),
     "all" (every period's V) or "none". `tile` 0 picks each launch's tile from
     `tile_table` (default: the table measured on this GPU model).
+    `order="longest"` runs each launch's tasks longest first on every XCD
+    (estimated work: states x actions per state), so the launch does not end
+    waiting on a long task that started last; "admission" keeps request order.
     As with any spawned pool, a main script using `lower_procs` must guard its
     entry point with `if __name__ == "__main__":`.
     """
@@ -887,7 +890,9 @@ class NativeSolveServer:
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_procs: int = 4, lower_threads: int = 2,
                  tile_table: dict | None = None, lanes: int = 1, queue_depth: int = 2, placers: int = 2,
-                 result_pinned_bytes: int = 64 << 20):
+                 result_pinned_bytes: int = 64 << 20, order: str = "longest"):
+        if order not in ("longest", "admission"):
+            raise ValueError(f"order must be 'longest' or 'admission', not {order!r}")
         if fetch not in ("first", "all", "none"):
             raise ValueError(f"fetch must be 'first', 'all' or 'none', not {fetch!r}")
         self.dt = np.dtype(dtype)
@@ -909,7 +914,7 @@ class NativeSolveServer:
         nat.call("lcm_serve_create", C.byref(self._h), self.code, C.c_size_t(budget), int(max_desc),
                  int(max_tasks), int(max_inflight), C.c_size_t(int(result_pinned_bytes)), self.n_lanes,
                  int(queue_depth), int(placers), int(tile), self._t_units.ctypes.data, self._t_acts.ctypes.data,
-                 self._t_tiles.ctypes.data, len(ent))
+                 self._t_tiles.ctypes.data, len(ent), nat.SERVE_LONGEST_FIRST if order == "longest" else 0)
         self.budget = budget
         self._spill = _spill_prefix()
         self._lock = threading.Lock()

@@ -48,13 +48,13 @@ def _check(handles, reqs, tile=256, periods="all", dtype=np.float32):
                 assert np.array_equal(h.result[t][r], ref[t][r]), (h.rid, t, r)
 
 
-@pytest.mark.parametrize("lanes", [1, 2])
-def test_native_results_same_alone_and_in_the_mix(native, lanes):
+@pytest.mark.parametrize("lanes,order", [(1, "admission"), (1, "longest"), (2, "longest")])
+def test_native_results_same_alone_and_in_the_mix(native, lanes, order):
     from pylcm_amd.serving import NativeSolveServer
 
     reqs = _requests(24, seed=lanes)
     server = NativeSolveServer(tile=128, fetch="all", budget_bytes=1 << 30, lower_procs=0, lower_threads=2,
-                               lanes=lanes)
+                               lanes=lanes, order=order)
     try:
         handles = [server.submit(m, p) for m, p in reqs[:12]]
         handles[0].done.wait(60)
