@@ -326,7 +326,15 @@ int lcm_philox(lcm_ctx* ctx, uint32_t* d_out, int n, uint32_t c1, uint32_t c2, u
 /* lcm_gs_launch with an explicit kernel variant: 0 = operands staged in LDS,
  * linear next-state grids and interval constraints (every descriptor must
  * have use_interval set and g_kind linear); 1 = staged, any grid, constraint
- * rows; 2 = general (tables may vary with the continuous state). */
+ * rows; 2 = general (tables may vary with the continuous state). One of the
+ * LCM_GS_OCC_* flags may be or'ed in to choose the float32 kernel's register
+ * budget: DENSE is compiled for 8 waves per SIMD (for launches that fill every
+ * CU several times over), SPARSE unconstrained (fewer, faster waves, for
+ * small launches); without a flag, launches of LCM_GS_DENSE_MIN_TASKS tasks
+ * or more run DENSE. Both give the same bits; float64 has one build. */
+#define LCM_GS_OCC_DENSE 0x100
+#define LCM_GS_OCC_SPARSE 0x200
+#define LCM_GS_DENSE_MIN_TASKS 6144
 int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
                      const LcmGsTask* d_tasks, int n_tasks, int tile,
                      size_t lds_bytes, int variant);

@@ -611,9 +611,17 @@ class GsProgram:
         st = L.lcm_gs_launch_ex(
             self.ctx.handle, code, C.c_void_p(self.base + self.desc_off),
             C.c_void_p(self.base + self.task_off[s]), self.n_tasks[s], self.tile,
-            C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]),
+            C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]) | self._occ_flag(),
         )
         nat._check(st, f"lcm_gs_launch (step {s})")
+
+    # the float32 kernel's register budget: "auto" lets the library pick by
+    # the launch's task count; "dense" / "sparse" force one build (the same
+    # bits either way)
+    occupancy = "auto"
+
+    def _occ_flag(self) -> int:
+        return {"auto": 0, "dense": nat.GS_OCC_DENSE, "sparse": nat.GS_OCC_SPARSE}[self.occupancy]
 
     def launch_regime(self, p: int, t: int, r: str):
         """One (problem, period, regime) launch: that descriptor's tasks alone,
@@ -637,7 +645,7 @@ class GsProgram:
         code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         st = nat.lib().lcm_gs_launch_ex(
             self.ctx.handle, code, C.c_void_p(self.base + self.desc_off), C.c_void_p(buf.ptr), npad,
-            self.tile, C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]))
+            self.tile, C.c_size_t(self.lds_by_step[s]), int(self.wdep_by_step[s]) | self._occ_flag())
         nat._check(st, f"lcm_gs_launch ({r!r}, period {t})")
 
     def _record(self):

@@ -28,6 +28,9 @@ REQ_MAGIC = 0x31305145524D434C  # "LCMREQ01", the request files of the native so
 REQ_TILE = 256
 DESC_PTR_WORDS = 36
 SERVE_LONGEST_FIRST = 1
+GS_OCC_DENSE = 0x100  # or'ed into lcm_gs_launch_ex's variant: the float32 kernel's register budget
+GS_OCC_SPARSE = 0x200
+GS_DENSE_MIN_TASKS = 6144
 
 vp = C.c_void_p
 

@@ -555,8 +555,23 @@ __device__ __forceinline__ void dispatch_loop(const LoopArgs<T>& L, int mode, bo
   }
 }
 
-template <typename T, int TILE, int VAR>
-__global__ void __launch_bounds__(kBlock)
+// Waves per SIMD the register allocation must leave room for. DENSE: the
+// float32 kernel is compiled into 64 VGPRs (the SGPRs it spills go to VGPR
+// lanes), so a SIMD holds 8 waves instead of the 6 its unconstrained 76 VGPRs
+// allow: 9 % less time per launch on the 1024-draw batch, whose launches
+// fill every CU many times over. Each wave runs longer with fewer registers,
+// so a launch too small to fill the CUs is faster with the unconstrained
+// kernel; the host picks per launch by its workgroup count (on the 1024-draw
+// workload's shapes the two builds break even near 4,400 workgroups, the
+// dense one is 6 % faster at 8,600 and 9 % at 33,800, and 10-25 % slower
+// below 2,000). The float64 kernel
+// needs about 125 VGPRs and is compiled unconstrained only. Both builds run
+// the same arithmetic and give the same bits.
+template <typename T, bool DENSE>
+constexpr int kWavesPerEu = (DENSE && sizeof(T) == 4) ? 8 : 1;
+
+template <typename T, int TILE, int VAR, bool DENSE>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWavesPerEu<T, DENSE>)))
 gs_period_kernel(const LcmGsDesc* __restrict__ descs,
                  const LcmGsTask* __restrict__ tasks, int n_tasks) {
   constexpr bool WDEP = VAR == kVarWdep;
@@ -863,11 +878,11 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   }
 }
 
-template <typename T, int TILE>
-int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, size_t lds, int var) {
-  auto kern = var == kVarFast     ? gs_period_kernel<T, TILE, kVarFast>
-              : var == kVarStaged ? gs_period_kernel<T, TILE, kVarStaged>
-                                  : gs_period_kernel<T, TILE, kVarWdep>;
+template <typename T, int TILE, bool DENSE>
+int launch_occ(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, size_t lds, int var) {
+  auto kern = var == kVarFast     ? gs_period_kernel<T, TILE, kVarFast, DENSE>
+              : var == kVarStaged ? gs_period_kernel<T, TILE, kVarStaged, DENSE>
+                                  : gs_period_kernel<T, TILE, kVarWdep, DENSE>;
   if (lds > 64 * 1024) {
     LCM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
@@ -877,12 +892,21 @@ int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, siz
   return LCM_OK;
 }
 
+template <typename T, int TILE>
+int launch_tile(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, size_t lds, int var, bool dense) {
+  if constexpr (sizeof(T) == 4) {
+    if (dense) return launch_occ<T, TILE, true>(ctx, d, t, n, lds, var);
+  }
+  return launch_occ<T, TILE, false>(ctx, d, t, n, lds, var);
+}
+
 template <typename T>
-int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int tile, size_t lds, int var) {
+int launch_t(lcm_ctx* ctx, const LcmGsDesc* d, const LcmGsTask* t, int n, int tile, size_t lds, int var,
+             bool dense) {
   switch (tile) {
-    case 64: return launch_tile<T, 64>(ctx, d, t, n, lds, var);
-    case 128: return launch_tile<T, 128>(ctx, d, t, n, lds, var);
-    case 256: return launch_tile<T, 256>(ctx, d, t, n, lds, var);
+    case 64: return launch_tile<T, 64>(ctx, d, t, n, lds, var, dense);
+    case 128: return launch_tile<T, 128>(ctx, d, t, n, lds, var, dense);
+    case 256: return launch_tile<T, 256>(ctx, d, t, n, lds, var, dense);
     default: return fail(LCM_E_ARG, "tile must be 64, 128 or 256");
   }
 }
@@ -907,9 +931,17 @@ extern "C" int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_desc
   if (n_tasks <= 0 || n_tasks % kXcds != 0)
     return fail(LCM_E_ARG, "lcm_gs_launch: n_tasks must be a positive multiple of 8");
   if (lds_bytes > 160 * 1024) return fail(LCM_E_ARG, "lcm_gs_launch: LDS request above 160 KiB");
+  const int occ = variant & (LCM_GS_OCC_DENSE | LCM_GS_OCC_SPARSE);
+  variant &= ~(LCM_GS_OCC_DENSE | LCM_GS_OCC_SPARSE);
   if (variant < kVarFast || variant > kVarWdep) return fail(LCM_E_ARG, "lcm_gs_launch: unknown variant");
-  if (dtype == LCM_DTYPE_F32) return launch_t<float>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant);
-  if (dtype == LCM_DTYPE_F64) return launch_t<double>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant);
+  if (occ == (LCM_GS_OCC_DENSE | LCM_GS_OCC_SPARSE))
+    return fail(LCM_E_ARG, "lcm_gs_launch: both register budgets requested");
+  // the 8-waves-per-SIMD build for launches that fill the CUs many times over
+  const bool dense = occ ? occ == LCM_GS_OCC_DENSE : n_tasks >= LCM_GS_DENSE_MIN_TASKS;
+  if (dtype == LCM_DTYPE_F32)
+    return launch_t<float>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant, dense);
+  if (dtype == LCM_DTYPE_F64)
+    return launch_t<double>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant, false);
   return fail(LCM_E_ARG, "lcm_gs_launch: unknown dtype");
 }
 
