@@ -331,7 +331,8 @@ int lcm_philox(lcm_ctx* ctx, uint32_t* d_out, int n, uint32_t c1, uint32_t c2, u
  * budget: DENSE is compiled for 8 waves per SIMD (for launches that fill every
  * CU several times over), SPARSE unconstrained (fewer, faster waves, for
  * small launches); without a flag, launches of LCM_GS_DENSE_MIN_TASKS tasks
- * or more run DENSE. Both give the same bits; float64 has one build. */
+ * or more whose lds_bytes leave room for more than 6 workgroups per compute
+ * unit run DENSE. Both give the same bits; float64 has one build. */
 #define LCM_GS_OCC_DENSE 0x100
 #define LCM_GS_OCC_SPARSE 0x200
 #define LCM_GS_DENSE_MIN_TASKS 6144

@@ -937,7 +937,13 @@ extern "C" int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_desc
   if (occ == (LCM_GS_OCC_DENSE | LCM_GS_OCC_SPARSE))
     return fail(LCM_E_ARG, "lcm_gs_launch: both register budgets requested");
   // the 8-waves-per-SIMD build for launches that fill the CUs many times over
-  const bool dense = occ ? occ == LCM_GS_OCC_DENSE : n_tasks >= LCM_GS_DENSE_MIN_TASKS;
+  // and whose LDS lets more than the unconstrained build's 6 waves share a
+  // SIMD (a workgroup is 4 waves, one per SIMD: the workgroups that fit in a
+  // CU's LDS are the waves per SIMD); with fewer the dense build only runs
+  // each wave slower
+  const size_t lds_per_wg = lds_bytes > 0 ? lds_bytes : 1;
+  const bool room = (size_t(160) * 1024) / lds_per_wg > 6;
+  const bool dense = occ ? occ == LCM_GS_OCC_DENSE : (n_tasks >= LCM_GS_DENSE_MIN_TASKS && room);
   if (dtype == LCM_DTYPE_F32)
     return launch_t<float>(ctx, d_descs, d_tasks, n_tasks, tile, lds_bytes, variant, dense);
   if (dtype == LCM_DTYPE_F64)

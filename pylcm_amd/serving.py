@@ -155,10 +155,16 @@ def _unspill(data: dict) -> np.ndarray:
 def _pool_worker_main(conn, spill):
     """A lowering process: receive (rid, dtype, tile, blob), send back
     (rid, data) or (rid, exception), until told to stop (None)."""
+    import gc
+
     import cloudpickle  # noqa: F401  (imported once, before the first request)
 
     from pylcm_amd import examples  # noqa: F401
 
+    # what the imports built lives as long as the worker: keep it out of the
+    # cyclic collector's scans, so a collection pauses a request for less
+    gc.collect()
+    gc.freeze()
     conn.send(("ready", None))
     while True:
         try:
