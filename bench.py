@@ -376,7 +376,7 @@ def simulation_leg(args):
     return out
 
 
-def _serve_window(args, rate, seconds, seed):
+def _serve_window(args, rate, seconds, seed, sim_agents=0):
     from pylcm_amd import examples as ex
     from pylcm_amd.serving import SolveServer, serve_poisson
 
@@ -391,11 +391,18 @@ def _serve_window(args, rate, seconds, seed):
         server = NativeSolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first",
                                    lower_procs=args.serve_procs, lanes=args.serve_lanes,
                                    queue_depth=args.serve_depth, placers=args.serve_placers,
-                                   order=args.serve_order)
+                                   order=args.serve_order, sim_threads=args.serve_sim_threads)
     else:
         server = SolveServer(dtype=np.float32, tile=args.serve_tile, fetch="first", lower_threads=args.serve_threads,
                              lower_procs=args.serve_procs, lanes=args.serve_lanes, queue_depth=args.serve_depth)
-    make = ex.serving_request_factory(seed=7)
+    base_make = ex.serving_request_factory(seed=7)
+    if sim_agents > 0:  # every request also simulates sim_agents agents on its solution
+        def make(i):
+            m, p = base_make(i)
+            return m, p, {"n_agents": sim_agents, "seed": 1000 + i,
+                          "initial_conditions": ex.serving_initial_conditions}
+    else:
+        make = base_make
     # the client keeps every request; none of them forms a reference cycle, so
     # the cyclic collector only adds pauses while the window runs
     import gc
@@ -422,6 +429,7 @@ def _serve_window(args, rate, seconds, seed):
            "mean_tasks_per_launch": round(server.launch_tasks / max(server.launches, 1), 1),
            "launches_by_tile": {str(k): v for k, v in sorted(server.tiles_used.items())},
            "loop": args.serve_loop, "lanes": args.serve_lanes, "client_gc": "disabled during the window",
+           "simulated_agents_per_request": sim_agents,
            "task_order": args.serve_order if args.serve_loop == "native" else "admission"}
     native = args.serve_loop == "native"
     host = server.host_seconds() if native else dict(server.timing)
@@ -431,7 +439,9 @@ def _serve_window(args, rate, seconds, seed):
     stages = {"lower": [r.t_lowered - r.t_submit for r in ok],
               "place": [r.t_admit - r.t_lowered for r in ok],
               "solve": [(r.t_retired if native else r.t_done) - r.t_admit for r in ok]}
-    if native:  # completion in the library to the result in the client's hands
+    if native and sim_agents > 0:  # solve retired in the library to the simulated panel in hand
+        stages["simulate"] = [r.t_done - r.t_retired for r in ok]
+    elif native:  # completion in the library to the result in the client's hands
         stages["collect"] = [r.t_done - r.t_retired for r in ok]
     out["stage_ms"] = {k: {"p50": round(1e3 * float(np.percentile(v, 50)), 2),
                            "p99": round(1e3 * float(np.percentile(v, 99)), 2)} for k, v in stages.items()}
@@ -464,11 +474,14 @@ def serving_leg(args):
     from pylcm_amd._gs_program import GsProgram
 
     dt = np.float32
-    win = _serve_window(args, args.serve_rate, args.serve_seconds, seed=11)
-    sat = _serve_window(args, args.serve_sat_rate, args.serve_sat_seconds, seed=12) if args.serve_sat_seconds > 0 else None
-    # plain: the same sequence, one request at a time (lower, place, solve, fetch period 0)
+    # plain: the same sequence, one request at a time (lower, place, solve,
+    # fetch period 0), timed first so that nothing the windows leave behind
+    # (their garbage, worker processes winding down) lands in it
+    import gc
+
+    gc.collect()
     make2 = ex.serving_request_factory(seed=7)
-    n_plain = min(args.serve_plain_sample, win["requests"])
+    n_plain = args.serve_plain_sample
     plain = []
     for i in range(n_plain):
         m, p = make2(i)
@@ -484,12 +497,18 @@ def serving_leg(args):
                 nat.d2h(pr.ctx, pr.base + oo.v_off, v)
         plain.append(time.perf_counter() - t_a)
         del pr
+    win = _serve_window(args, args.serve_rate, args.serve_seconds, seed=11,
+                        sim_agents=args.serve_sim_agents if args.serve_loop == "native" else 0)
+    sat = _serve_window(args, args.serve_sat_rate, args.serve_sat_seconds, seed=12) if args.serve_sat_seconds > 0 else None
     plain = np.array(plain)
     plain_sps = 1.0 / plain.mean()
     return {
         "workload": f"Poisson arrivals {args.serve_rate:g}/s for {args.serve_seconds:g}s; deterministic and AR(1)-income "
                     "retirement models, T U{20..60}, grids U{200..4000}, float32, period-0 V returned; "
-                    f"requests lowered in {args.serve_procs} worker processes, {args.serve_loop} serving loop",
+                    f"requests lowered in {args.serve_procs} worker processes, {args.serve_loop} serving loop; "
+                    f"each request of the {args.serve_rate:g}/s window also forward-simulates "
+                    f"{win['simulated_agents_per_request']} agents on its resident solution; the saturation "
+                    "window solves only",
         **win,
         "saturation": sat,
         "plain_serial": {"solves_per_s": round(plain_sps, 2),
@@ -584,6 +603,9 @@ def main():
     ap.add_argument("--serve-loop", choices=("native", "python"), default="native",
                     help="serving loop in library threads (native) or in this process (python)")
     ap.add_argument("--serve-placers", type=int, default=2, help="library threads placing requests (native loop)")
+    ap.add_argument("--serve-sim-agents", type=int, default=10_000,
+                    help="agents each request of the arrival-rate window simulates (native loop; 0: none)")
+    ap.add_argument("--serve-sim-threads", type=int, default=4, help="simulation threads (streams) of the server")
     ap.add_argument("--serve-order", choices=("longest", "admission"), default="longest",
                     help="order of a fused launch's tasks (native loop)")
     ap.add_argument("--serve-depth", type=int, default=2, help="fused launches queued per lane")

@@ -383,6 +383,7 @@ int lcm_event_elapsed_batch(void* ref, void* const* evs, int n, float* out_ms);
  * base + out_start). A request occupies total_bytes of device memory.
  */
 #define LCM_REQ_MAGIC 0x31305145524d434cULL /* "LCMREQ01" */
+#define LCM_REQ_HOLD 1 /* flags: keep the request's device memory after its solve (lcm_serve_release) */
 #define LCM_REQ_TILE 256                    /* states per task row */
 #define LCM_DESC_PTR_WORDS 36               /* leading pointer words of LcmGsDesc */
 
@@ -390,7 +391,8 @@ typedef struct LcmReqHeader {
   uint64_t magic;
   int64_t dtype, n_steps, n_desc, n_task, n_fetch;
   int64_t image_bytes, out_start, total_bytes, tile;
-  int64_t reserved[6];
+  int64_t flags; /* LCM_REQ_HOLD or 0 */
+  int64_t reserved[5];
 } LcmReqHeader;
 
 /* Parse and check a request file without a device: info[0..8) = n_steps,
@@ -451,6 +453,14 @@ int lcm_serve_wait(lcm_serve* s, int64_t* rids, int32_t* status, int max_n, int 
  * id). */
 int lcm_serve_fetch(lcm_serve* s, int64_t rid, void* dst, size_t nbytes, double* times, char* err,
                     size_t err_len);
+
+/* A completed request submitted with LCM_REQ_HOLD keeps its device memory
+ * (value functions, tables and descriptors stay valid, the solve's work is
+ * done) until lcm_serve_release: *base = its first byte, *descs = its first
+ * descriptor record in the server's pool; for device work on the solved
+ * request, such as a forward simulation. */
+int lcm_serve_held(lcm_serve* s, int64_t rid, uint64_t* base, uint64_t* descs);
+int lcm_serve_release(lcm_serve* s, int64_t rid);
 
 /* Counters: out[0..12) = launches, task records launched, launches at tile
  * 64, 128, 256, nanoseconds the lanes spent assembling and issuing launches,

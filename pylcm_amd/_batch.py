@@ -52,9 +52,10 @@ def pointer_classes(D0: np.ndarray, masks: dict, out_start: int) -> np.ndarray:
     return cls
 
 
-def pack_problem(model, params, dtype, tile: int) -> dict:
+def pack_problem(model, params, dtype, tile: int, keep_lowered: bool = False) -> dict:
     """Lower and pack one (model, params) on the host (no device code):
-    everything needed to place and run it anywhere in device memory."""
+    everything needed to place and run it anywhere in device memory. With
+    `keep_lowered` the result also holds the lowered (tables, grids)."""
     dt = np.dtype(dtype)
     low = lower_problem(model, params, dt)
     prog = GsProgram.of([(model, params)], dt, lowered=[low], tile=tile, defer=True)
@@ -67,7 +68,7 @@ def pack_problem(model, params, dtype, tile: int) -> dict:
             "tasks_host": prog.tasks_host, "outs": prog.outs, "n_steps": prog.n_steps,
             "lds_g": prog._lds_g, "lds_k": prog._lds_k, "wdep_by_step": prog.wdep_by_step,
             "total_bytes": prog.total_bytes, "out_start": prog.out_start, "n_desc": prog.n_desc,
-            "flops_by_step": prog.flops_by_step}
+            "flops_by_step": prog.flops_by_step, **({"lowered": low} if keep_lowered else {})}
 
 
 # -- worker processes -------------------------------------------------------------

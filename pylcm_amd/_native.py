@@ -26,6 +26,7 @@ MAX_TASTE_D = 8
 ACT_CHUNK = 256
 REQ_MAGIC = 0x31305145524D434C  # "LCMREQ01", the request files of the native solve server
 REQ_TILE = 256
+REQ_HOLD = 1
 DESC_PTR_WORDS = 36
 SERVE_LONGEST_FIRST = 1
 GS_OCC_DENSE = 0x100  # or'ed into lcm_gs_launch_ex's variant: the float32 kernel's register budget
@@ -209,6 +210,8 @@ _SIGS = {
     "lcm_serve_submit": ([vp, C.c_int64, C.c_char_p], C.c_int),
     "lcm_serve_wait": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int, C.c_int], C.c_int),
     "lcm_serve_fetch": ([vp, C.c_int64, vp, C.c_size_t, C.POINTER(C.c_double), C.c_char_p, C.c_size_t], C.c_int),
+    "lcm_serve_held": ([vp, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)], C.c_int),
+    "lcm_serve_release": ([vp, C.c_int64], C.c_int),
     "lcm_serve_stats": ([vp, C.POINTER(C.c_int64), C.c_int], C.c_int),
     "lcm_serve_busy_ms": ([vp, C.POINTER(C.c_double)], C.c_int),
     "lcm_serve_destroy": ([vp], C.c_int),

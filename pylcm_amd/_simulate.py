@@ -518,21 +518,158 @@ class SimProgram:
         if len(model.regime_names) > nat.SIM_MAX_REGIMES:
             raise UnsupportedModelError(f"simulation supports at most {nat.SIM_MAX_REGIMES} regimes")
 
+    @classmethod
+    def for_placement(cls, model, params, prog):
+        """A program that only places an image `build_image` made elsewhere
+        (in a lowering process): `prog` provides the stream (`ctx`), `dt`
+        and the solve's descriptors at `base + desc_off`."""
+        self = cls.__new__(cls)
+        self.static_scratch = False
+        self.model = model
+        self.params = params
+        self.prog = prog
+        self.dt = prog.dt
+        self.sl = {r: SimLayout(model.layouts[r]) for r in model.regime_names}
+        if len(model.regime_names) > nat.SIM_MAX_REGIMES:
+            raise UnsupportedModelError(f"simulation supports at most {nat.SIM_MAX_REGIMES} regimes")
+        return self
+
     def run(self, ic, seed):
         self.prepare(ic, seed)
         self.launch()
         self.prog.ctx.sync()
         return self.collect(ic)
 
-    def prepare(self, ic, seed):
+    def prepare(self, ic, seed, mem=None):
         """Allocate the agent panels, lower every period's tables and upload
-        them with the descriptors; `launch` then enqueues the simulation."""
+        them with the descriptors; `launch` then enqueues the simulation.
+        `build_image` (host only) followed by `place` (device)."""
+        self.place(self.build_image(), ic, seed, mem=mem)
+
+    # pointer bases of a simulation image: its own table arena, the arena of
+    # EV rows / CS / finite flags the device writes, the solve's descriptors
+    BASE_TABLES, BASE_EV, BASE_SOLVE = 0, 1, 2
+
+    def build_image(self) -> dict:
+        """The host half of `prepare`: every period's simulation tables and
+        the EV and simulation descriptors, with each pointer written relative
+        to one of three bases and listed by byte offset in `patch[kind]`, so
+        the image can be built where the solve was lowered (another process)
+        and placed anywhere. The agent panels are not part of it."""
         model, dt, prog = self.model, self.dt, self.prog
         T = model.n_periods
-        n = ic["_n"]
         isz = dt.itemsize
         names = list(model.regime_names)
         R = len(names)
+        from pylcm_amd._gs_program import _Arena, DESC_DTYPE
+
+        ar = _Arena()  # tables and descriptors (uploaded)
+        rs = _Arena()  # EV rows, CS and per-row finite flags (written on the device)
+        zero = ar.add(np.zeros(4, dtype=dt))
+        ev_meta = {}
+        for (t, r), o in prog.outs[0].items():
+            lay = model.layouts[r]
+            pt = prog.problems[0][(t, r)]
+            if lay.terminal:
+                continue
+            # g_n = 0: only stateless targets, the kernel fills CS alone
+            g_n = len(self.rg.w_pts[pt.next_grid]) if pt.targets else 0
+            ev_meta[(t, r)] = (rs.reserve(max(lay.n_sd * lay.n_d * g_n, 1) * isz),
+                               rs.reserve(lay.n_sd * lay.n_d * isz), g_n,
+                               rs.reserve(lay.n_sd * lay.n_d * 4))  # per-row finite flags
+        sim_desc = {}
+        lds_max = 0
+        keep_tables = []  # the arena dedupes by object identity while it is built
+        L = nat.load_library()
+        code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
+        for t in range(T):
+            for r in model.active_regimes(t):
+                sl = self.sl[r]
+                lay = sl.lay
+                st = self.low.lower_sim(sl, t)
+                keep_tables.append(st)
+                sim_desc[(t, r)] = self._desc(st, sl, t, r, ar, zero, ev_meta)
+                g_n = ev_meta.get((t, r), (0, 0, 0, 0))[2]
+                ntab = st.n_sdd * (7 + 4 * len(st.feats))
+                need = int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab))
+                # EV rows are read in place from global memory when one cell's
+                # rows fit in a CU's L1 (no staging pass) or do not fit in LDS
+                # next to the tables; otherwise they are staged in LDS
+                cell_bytes = lay.n_sd * g_n * dt.itemsize
+                ev_global = need > SIM_LDS_LIMIT or cell_bytes <= SIM_EV_L1_BYTES
+                sim_desc[(t, r)]["ev_global"] = int(ev_global)
+                if ev_global:
+                    need = int(L.lcm_sim_lds_bytes(code, g_n, 0, ntab))
+                lds_max = max(lds_max, need)
+        if lds_max > SIM_LDS_LIMIT:
+            raise UnsupportedModelError("simulation action tables and next-state grid do not fit in LDS "
+                                        f"({lds_max} B > {SIM_LDS_LIMIT} B)")
+        # EV descriptors point at the solve program's own descriptors
+        gs_index = self._gs_desc_index()
+        ev_dt = np.dtype(nat.LcmEvDesc)
+        sd_dt = np.dtype(nat.LcmSimDesc)
+        ev_descs = np.zeros(len(ev_meta), dtype=ev_dt)
+        ev_keys = sorted(ev_meta)
+        ev_desc_off = _align(ar.size)
+        sim_desc_off = ev_desc_off + _align(len(ev_keys) * ev_dt.itemsize)
+        patch = {self.BASE_TABLES: [], self.BASE_EV: [], self.BASE_SOLVE: []}
+
+        for i, key in enumerate(ev_keys):
+            t, r = key
+            lay = model.layouts[r]
+            e_off, c_off, g_n, f_off = ev_meta[key]
+            ev_descs[i]["gs"] = gs_index[key] * DESC_DTYPE.itemsize
+            ev_descs[i]["EV"], ev_descs[i]["CS"], ev_descs[i]["fin"] = e_off, c_off, f_off
+            ev_descs[i]["n_sd"], ev_descs[i]["n_d"], ev_descs[i]["g_n"] = lay.n_sd, lay.n_d, g_n
+            base_i = ev_desc_off + i * ev_dt.itemsize
+            patch[self.BASE_SOLVE].append(base_i + ev_dt.fields["gs"][1])
+            for f in ("EV", "CS", "fin"):
+                patch[self.BASE_EV].append(base_i + ev_dt.fields[f][1])
+        SD = np.zeros(T * R, dtype=sd_dt)
+        for (t, r), d in sim_desc.items():
+            i = t * R + model.regime_code(r)
+            rec = SD[i]
+            base_i = sim_desc_off + i * sd_dt.itemsize
+            for k, v in d.items():
+                if k in ("Uf", "Bf", "fcoef", "fconst", "Wd", "LoF", "HiF"):
+                    for j, p in enumerate(v):
+                        rec[k][j] = p if p is not None else 0
+                        if p is not None:
+                            patch[self.BASE_TABLES].append(base_i + sd_dt.fields[k][1] + 8 * j)
+                elif k in ("n_sdd_next", "feat_op"):
+                    for j, p in enumerate(v):
+                        rec[k][j] = p
+                elif k in ("U0", "A", "B0", "lo", "hi", "sd_base", "nodes", "g_pts", "preg", "LoB", "HiB"):
+                    rec[k] = v if v is not None else 0
+                    if v is not None:
+                        patch[self.BASE_TABLES].append(base_i + sd_dt.fields[k][1])
+                elif k in ("EV", "CS", "ev_fin"):
+                    rec[k] = v if v is not None else 0
+                    if v is not None:
+                        patch[self.BASE_EV].append(base_i + sd_dt.fields[k][1])
+                else:
+                    rec[k] = v
+        ar.parts.append((ev_desc_off, ev_descs.tobytes()))
+        ar.parts.append((sim_desc_off, SD.tobytes()))
+        ar.size = max(ar.size, sim_desc_off + SD.nbytes)
+        del keep_tables
+        return {"T": T, "R": R, "image": ar.image(), "ev_bytes": _align(max(rs.size, 1)),
+                "patch": {k: np.asarray(v, dtype=np.int64) for k, v in patch.items()},
+                "ev_desc_off": ev_desc_off, "sim_desc_off": sim_desc_off, "ev_keys": ev_keys,
+                "lds_max": lds_max, "sim_desc_size": sd_dt.itemsize}
+
+    def place(self, img: dict, ic, seed, mem=None):
+        """The device half of `prepare`: allocate the agent panels (n = the
+        initial conditions' agents) and the image's two arenas, write the
+        initial state, relocate the image's pointers and upload it. `mem`,
+        when given, is `mem(slot, nbytes) -> device pointer` (slot 0: panels,
+        1: tables and EV rows), so a caller can reuse its own memory;
+        otherwise the program allocates its buffers."""
+        model, dt, prog = self.model, self.dt, self.prog
+        T, R = img["T"], img["R"]
+        n = ic["_n"]
+        isz = dt.itemsize
+        names = list(model.regime_names)
         # -- device buffers ------------------------------------------------------
         parts = []
 
@@ -548,9 +685,14 @@ class SimProgram:
                           w=alloc(T * n * isz), z=alloc(T * n * isz), sdd=alloc(T * n * 4))
         off["counts"] = alloc(2 * R * 4)  # two halves, alternating by period
         off["order"] = alloc(R * n * 4)
-        # per-period tables, EV buffers and descriptors go into a second arena
-        self.buf = nat.DeviceBuffer(sum(_align(p) for p in parts))
-        base = self.buf.ptr
+        panel_bytes = sum(_align(p) for p in parts)
+        arena_bytes = _align(img["image"].nbytes) + img["ev_bytes"]
+        if mem is None:
+            self.buf = nat.DeviceBuffer(panel_bytes)
+            self.buf2 = nat.DeviceBuffer(arena_bytes)
+            base, b2 = self.buf.ptr, self.buf2.ptr
+        else:
+            base, b2 = mem(0, panel_bytes), mem(1, arena_bytes)
         ctx = prog.ctx
         # -- initial agent state -------------------------------------------------
         w0 = np.zeros(n)
@@ -580,89 +722,16 @@ class SimProgram:
         nat.h2d(ctx, base + off["cur"]["regime"], reg0)
         nat.h2d(ctx, base + off["cur"]["start"], ic["_start"])
         nat.h2d(ctx, base + off["nxt"]["start"], ic["_start"])
-        # -- per-period lowering, EV and sim descriptors --------------------------
-        from pylcm_amd._gs_program import _Arena, DESC_DTYPE
-
-        ar = _Arena()
-        zero = ar.add(np.zeros(4, dtype=dt))
-        ev_meta = {}
-        for (t, r), o in prog.outs[0].items():
-            lay = model.layouts[r]
-            pt = prog.problems[0][(t, r)]
-            if lay.terminal:
-                continue
-            # g_n = 0: only stateless targets, the kernel fills CS alone
-            g_n = len(self.rg.w_pts[pt.next_grid]) if pt.targets else 0
-            ev_meta[(t, r)] = (ar.reserve(max(lay.n_sd * lay.n_d * g_n, 1) * isz),
-                               ar.reserve(lay.n_sd * lay.n_d * isz), g_n,
-                               ar.reserve(lay.n_sd * lay.n_d * 4))  # per-row finite flags
-        sim_desc = {}
-        lds_max = 0
-        keep_tables = []
-        for t in range(T):
-            for r in model.active_regimes(t):
-                sl = self.sl[r]
-                lay = sl.lay
-                st = self.low.lower_sim(sl, t)
-                keep_tables.append(st)
-                sim_desc[(t, r)] = self._desc(st, sl, t, r, ar, zero, ev_meta)
-                g_n = ev_meta.get((t, r), (0, 0, 0, 0))[2]
-                ntab = st.n_sdd * (7 + 4 * len(st.feats))
-                L = nat.load_library()
-                code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
-                need = int(L.lcm_sim_lds_bytes(code, g_n, lay.n_sd if g_n else 0, ntab))
-                # EV rows are read in place from global memory when one cell's
-                # rows fit in a CU's L1 (no staging pass) or do not fit in LDS
-                # next to the tables; otherwise they are staged in LDS
-                cell_bytes = lay.n_sd * g_n * dt.itemsize
-                ev_global = need > SIM_LDS_LIMIT or cell_bytes <= SIM_EV_L1_BYTES
-                sim_desc[(t, r)]["ev_global"] = int(ev_global)
-                if ev_global:
-                    need = int(L.lcm_sim_lds_bytes(code, g_n, 0, ntab))
-                lds_max = max(lds_max, need)
-        if lds_max > SIM_LDS_LIMIT:
-            raise UnsupportedModelError("simulation action tables and next-state grid do not fit in LDS "
-                                        f"({lds_max} B > {SIM_LDS_LIMIT} B)")
-        # EV descriptors point at the solve program's own descriptors
-        gs_index = self._gs_desc_index()
-        ev_descs = np.zeros(len(ev_meta), dtype=np.dtype(nat.LcmEvDesc))
-        ev_keys = sorted(ev_meta)
-        self.buf2 = nat.DeviceBuffer(_align(ar.size) + _align(len(ev_keys) * ev_descs.dtype.itemsize)
-                                     + _align(T * R * np.dtype(nat.LcmSimDesc).itemsize))
-        b2 = self.buf2.ptr
-        ev_desc_off = _align(ar.size)
-        sim_desc_off = ev_desc_off + _align(len(ev_keys) * ev_descs.dtype.itemsize)
-        for i, key in enumerate(ev_keys):
-            t, r = key
-            lay = model.layouts[r]
-            e_off, c_off, g_n, f_off = ev_meta[key]
-            ev_descs[i]["gs"] = prog.base + prog.desc_off + gs_index[key] * DESC_DTYPE.itemsize
-            ev_descs[i]["EV"] = b2 + e_off
-            ev_descs[i]["CS"] = b2 + c_off
-            ev_descs[i]["fin"] = b2 + f_off
-            ev_descs[i]["n_sd"], ev_descs[i]["n_d"], ev_descs[i]["g_n"] = lay.n_sd, lay.n_d, g_n
-        SD = np.zeros(T * R, dtype=np.dtype(nat.LcmSimDesc))
-        for (t, r), d in sim_desc.items():
-            rec = SD[t * R + model.regime_code(r)]
-            for k, v in d.items():
-                if k in ("Uf", "Bf", "fcoef", "fconst", "Wd", "LoF", "HiF"):
-                    for j, p in enumerate(v):
-                        rec[k][j] = (b2 + p) if p is not None else 0
-                elif k in ("n_sdd_next", "feat_op"):
-                    for j, p in enumerate(v):
-                        rec[k][j] = p
-                elif k in ("U0", "A", "B0", "lo", "hi", "EV", "CS", "ev_fin", "sd_base", "nodes", "g_pts", "preg",
-                           "LoB", "HiB"):
-                    rec[k] = (b2 + v) if v is not None else 0
-                else:
-                    rec[k] = v
-        ar.parts.append((ev_desc_off, ev_descs.tobytes()))
-        ar.parts.append((sim_desc_off, SD.tobytes()))
-        ar.size = max(ar.size, sim_desc_off + SD.nbytes)
-        nat.h2d(ctx, b2, ar.image())
-        self._st = dict(T=T, n=n, R=R, seed=seed, base=base, off=off, b2=b2, ev_desc_off=ev_desc_off,
-                        sim_desc_off=sim_desc_off, ev_keys=ev_keys, lds_max=lds_max,
-                        sim_desc_size=SD.dtype.itemsize, keep=keep_tables)
+        # -- the image, relocated ------------------------------------------------
+        ev_base = b2 + _align(img["image"].nbytes)
+        image = np.array(img["image"], dtype=np.uint8, copy=True)
+        words = image.view(np.uint64)
+        for kind, add in ((self.BASE_TABLES, b2), (self.BASE_EV, ev_base), (self.BASE_SOLVE, prog.base + prog.desc_off)):
+            words[img["patch"][kind] // 8] += np.uint64(add)
+        nat.h2d(ctx, b2, image)
+        self._st = dict(T=T, n=n, R=R, seed=seed, base=base, off=off, b2=b2, ev_desc_off=img["ev_desc_off"],
+                        sim_desc_off=img["sim_desc_off"], ev_keys=img["ev_keys"], lds_max=img["lds_max"],
+                        sim_desc_size=img["sim_desc_size"])
         # agent panels are rewritten by `launch`; keep the initial state to restore
         self._init_state = [(off[side][k], arr) for side, k, arr in (
             ("cur", "w", w0.astype(dt)), ("cur", "z", z0.astype(dt)), ("cur", "sdd", sdd0),
@@ -737,11 +806,11 @@ class SimProgram:
             if events:
                 events[1][t][1].record(ctx)
 
-    def collect(self, ic):
-        """Download the panel (after the stream has finished) and build the result."""
-        st, model, dt, ctx = self._st, self.model, self.dt, self.prog.ctx
+    def collect_raw(self) -> dict:
+        """The panel's [period, agent] arrays as the device wrote them (after
+        the stream has finished): regime, action, sdd (int32), value, w, z."""
+        st, dt, ctx = self._st, self.dt, self.prog.ctx
         T, n, base, off = st["T"], st["n"], st["base"], st["off"]
-        names = list(model.regime_names)
 
         def get(name, dtype_):
             arr = np.zeros(T * n, dtype=dtype_)
@@ -750,6 +819,13 @@ class SimProgram:
 
         out = {k: get(k, np.int32) for k in ("regime", "action", "sdd")}
         out.update({k: get(k, dt) for k in ("value", "w", "z")})
+        return out
+
+    def collect(self, ic):
+        """Download the panel (after the stream has finished) and build the result."""
+        model = self.model
+        names = list(model.regime_names)
+        out = self.collect_raw()
         self.raw = out
         res = SimulationResult(self._frame(out, ic), model,
                                frozenset(r for r in names if model.layouts[r].terminal),

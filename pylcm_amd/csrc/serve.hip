@@ -170,6 +170,7 @@ bool read_meta(int fd, ReqMeta& m, std::string& err) {
   if (h.magic != LCM_REQ_MAGIC) return err = "not a request file (bad magic)", false;
   if (h.dtype != LCM_DTYPE_F32 && h.dtype != LCM_DTYPE_F64) return err = "request dtype unknown", false;
   if (h.tile != LCM_REQ_TILE) return err = "request tasks must be 256-state tasks", false;
+  if (h.flags & ~int64_t(LCM_REQ_HOLD)) return err = "unknown request flags", false;
   const int64_t lim = int64_t(1) << 31;
   if (h.n_steps < 0 || h.n_steps > lim || h.n_desc < 0 || h.n_desc > lim || h.n_task < 0 ||
       h.n_task > lim || h.n_fetch < 0 || h.n_fetch > lim || h.image_bytes < 0 || h.out_start < 0 ||
@@ -243,6 +244,7 @@ struct Req {
   char* base = nullptr;
   size_t slab_off = 0, slab_len = 0, desc_off = 0, desc_len = 0, res_off = 0, res_len = 0;
   bool has_res = false;
+  bool hold = false;  // keep the slab and descriptors after the solve
   std::vector<int64_t> fetch_off, fetch_bytes;
   int lane = -1;
 };
@@ -298,6 +300,10 @@ struct lcm_serve {
   std::deque<Req*> submitted;
   std::deque<Req*> done_q;
   std::unordered_map<int64_t, Req*> finished;
+  struct Held {
+    size_t slab_off, slab_len, desc_off, desc_len;
+  };
+  std::unordered_map<int64_t, Held> held;  // completed requests keeping their memory
   int n_placed = 0;
   bool stop = false;
 
@@ -501,6 +507,7 @@ int lcm_serve::place(Req* r, char*& stage, size_t& stage_bytes, hipStream_t stre
     return unwind(LCM_E_LAUNCH);
   }
   r->base = base;
+  r->hold = (h.flags & LCM_REQ_HOLD) != 0;
   r->fetch_off = std::move(m.fetch_off);
   r->fetch_bytes = std::move(m.fetch_bytes);
   r->t_placed = now_s();
@@ -572,9 +579,13 @@ bool lcm_serve::retire_ready(Lane* L, bool block) {
     {
       std::lock_guard<std::mutex> lk(mu);
       for (Req* r : f.done) {
-        slab.release(r->slab_off, r->slab_len);
-        desc_slab.release(r->desc_off, r->desc_len);
-        --n_placed;
+        if (r->hold && q == hipSuccess) {
+          held[r->rid] = Held{r->slab_off, r->slab_len, r->desc_off, r->desc_len};
+        } else {
+          slab.release(r->slab_off, r->slab_len);
+          desc_slab.release(r->desc_off, r->desc_len);
+          --n_placed;
+        }
         --L->load;
         r->t_done = td;
         if (q != hipSuccess) {
@@ -957,6 +968,31 @@ int lcm_serve_fetch(lcm_serve* s, int64_t rid, void* dst, size_t nbytes, double*
   s->cv_mem.notify_all();
   delete r;
   return rc;
+}
+
+int lcm_serve_held(lcm_serve* s, int64_t rid, uint64_t* base, uint64_t* descs) {
+  if (!s || !base || !descs) return fail(LCM_E_ARG, "lcm_serve_held: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->held.find(rid);
+  if (it == s->held.end()) return fail(LCM_E_ARG, "lcm_serve_held: no held request of this id");
+  *base = reinterpret_cast<uint64_t>(s->slab_dev + it->second.slab_off);
+  *descs = reinterpret_cast<uint64_t>(reinterpret_cast<char*>(s->desc_dev) + it->second.desc_off);
+  return LCM_OK;
+}
+
+int lcm_serve_release(lcm_serve* s, int64_t rid) {
+  if (!s) return fail(LCM_E_ARG, "lcm_serve_release: null server");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->held.find(rid);
+    if (it == s->held.end()) return fail(LCM_E_ARG, "lcm_serve_release: no held request of this id");
+    s->slab.release(it->second.slab_off, it->second.slab_len);
+    s->desc_slab.release(it->second.desc_off, it->second.desc_len);
+    --s->n_placed;
+    s->held.erase(it);
+  }
+  s->cv_mem.notify_all();
+  return LCM_OK;
 }
 
 int lcm_serve_stats(const lcm_serve* s, int64_t* out, int n) {

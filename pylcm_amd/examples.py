@@ -600,6 +600,21 @@ def dcegm_params(n_periods, *, discount_factor=0.96, crra=2.0, disutility_of_wor
 # -- serving request mix ---------------------------------------------------------
 
 
+def serving_initial_conditions(model, params, grids, n_agents: int, rng):
+    """Initial conditions of a serving request's forward simulation: every
+    agent in working life with wealth lognormal(log 10, 0.5) and, in the
+    AR(1)-income model, the income node drawn from Binomial(nodes - 1, 1/2)
+    (the Rouwenhorst chain's stationary distribution); for
+    `NativeSolveServer.submit(..., simulate={"initial_conditions": ...})`."""
+    ic = {"wealth": np.exp(np.log(10.0) + 0.5 * rng.standard_normal(n_agents)),
+          "regime_id": np.zeros(n_agents, dtype=int)}
+    key = ("working_life", "income")
+    if key in grids.proc:
+        nodes = np.asarray(grids.proc[key][0], dtype=np.float64)
+        ic["income"] = nodes[rng.binomial(len(nodes) - 1, 0.5, n_agents)]
+    return ic
+
+
 def serving_request_factory(seed: int = 0, horizons=(20, 60), grids=(200, 4000), income_nodes: int = 7):
     """`make(i) -> (model, params)` drawing the serving workload's requests:
     half deterministic (the T=10 workload's economics), half with the AR(1)

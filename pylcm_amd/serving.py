@@ -66,7 +66,7 @@ def _lower_in_worker(blob: bytes, dtype: str, tile: int, spill: str | None = Non
     return data
 
 
-def write_request_file(path: str, data: dict, dtype, fetch: str = "first") -> list:
+def write_request_file(path: str, data: dict, dtype, fetch: str = "first", hold: bool = False) -> list:
     """Write one packed request (`pack_problem` output with 256-state tasks)
     as a request file of the native server (layout in `lcm_serve_submit`'s
     header: `LcmReqHeader`, per-step arrays, fetched outputs, pointer masks,
@@ -102,7 +102,7 @@ def write_request_file(path: str, data: dict, dtype, fetch: str = "first") -> li
     img = np.ascontiguousarray(data["image"]).view(np.uint8)
     code = nat.LCM_DTYPE_F64 if dt == np.float64 else nat.LCM_DTYPE_F32
     header = np.array([nat.REQ_MAGIC, code, n_steps, n, len(cat), len(keys), img.nbytes, data["out_start"],
-                       data["total_bytes"], nat.REQ_TILE] + [0] * 6, dtype=np.int64)
+                       data["total_bytes"], nat.REQ_TILE, nat.REQ_HOLD if hold else 0] + [0] * 5, dtype=np.int64)
     per_step = np.concatenate([np.asarray(data[k], dtype=np.int64).reshape(n_steps)
                                for k in ("task_counts", "lds_g", "lds_k", "wdep_by_step")]
                               + [np.asarray(f_off, dtype=np.int64), np.asarray(f_bytes, dtype=np.int64)])
@@ -115,20 +115,53 @@ def write_request_file(path: str, data: dict, dtype, fetch: str = "first") -> li
 def _lower_to_request_file(blob, dtype: str, spill: str, fetch: str) -> dict:
     """Lower one request in a worker and write it as a native request file
     (a path of the `spill` prefix); returns the path and the fetched outputs.
-    `blob` is the cloudpickled (model, params), or the pair itself."""
+    `blob` is the cloudpickled (model, params[, simulate]), or the tuple
+    itself. With a `simulate` spec the request is held after its solve and
+    the worker also writes `path + ".sim"`: the simulation image
+    (`SimProgram.build_image`), the initial conditions and the seed."""
     from pylcm_amd._batch import pack_problem
 
     t0 = time.perf_counter()
     if isinstance(blob, bytes):
         import cloudpickle
 
-        model, params = cloudpickle.loads(blob)
-    else:
-        model, params = blob
-    data = pack_problem(model, params, dtype, nat.REQ_TILE)
+        blob = cloudpickle.loads(blob)
+    model, params = blob[0], blob[1]
+    sim = blob[2] if len(blob) > 2 else None
+    data = pack_problem(model, params, dtype, nat.REQ_TILE, keep_lowered=sim is not None)
     path = f"{spill}{os.getpid()}_{threading.get_ident()}_{time.perf_counter_ns()}"
-    fetched = write_request_file(path, data, dtype, fetch)
-    return {"path": path, "fetched": fetched, "worker_s": time.perf_counter() - t0}
+    out = {"path": path}
+    if sim is not None:
+        out["sim_path"] = _write_sim_image(path + ".sim", model, params, data, dtype, sim)
+    out["fetched"] = write_request_file(path, data, dtype, fetch, hold=sim is not None)
+    out["worker_s"] = time.perf_counter() - t0
+    return out
+
+
+def _write_sim_image(path: str, model, params, data: dict, dtype, sim: dict) -> str:
+    """The host half of a request's forward simulation, written for the
+    server's simulation thread: `sim` = {"n_agents", "seed",
+    "initial_conditions"} where the initial conditions are a dict of arrays
+    or `f(model, params, grids, n_agents, rng) -> dict` (`grids` the resolved
+    grids, `rng` a NumPy generator seeded with `seed`)."""
+    import pickle
+    import types
+
+    from pylcm_amd._simulate import SimProgram, _initial_conditions
+
+    tabs, rg = data["lowered"]
+    view = types.SimpleNamespace(_rg={0: rg}, problems={0: tabs}, dt=np.dtype(dtype), outs=data["outs"])
+    img = SimProgram(model, params, view).build_image()
+    n, seed = int(sim["n_agents"]), int(sim.get("seed", 0))
+    ic_user = sim["initial_conditions"]
+    if callable(ic_user):
+        ic_user = ic_user(model, params, rg, n, np.random.default_rng(seed))
+    ic = _initial_conditions(model, ic_user)
+    if ic["_n"] != n:
+        raise ValueError(f"initial conditions hold {ic['_n']} agents, the spec asks for {n}")
+    with open(path, "wb") as f:
+        pickle.dump({"img": img, "ic": ic, "seed": seed}, f, protocol=pickle.HIGHEST_PROTOCOL)
+    return path
 
 
 def _unspill(data: dict) -> np.ndarray:
@@ -317,6 +350,10 @@ class Request:
     n_act_steps: list | None = None  # most actions per state of each step's tasks
     t_admit: float = 0.0
     t_retired: float = 0.0  # native server: its last launch retired (before collection)
+    t_simulated: float = 0.0  # native server: its forward simulation done
+    simulate: dict | None = None  # native server: forward-simulation spec
+    sim_path: str | None = None
+    simulation: dict | None = None  # [period, agent] arrays of the forward simulation
     t_done: float = 0.0
     fetched: list | None = None  # native server: [(period, regime, shape, bytes)] in file order
     slab: tuple | None = None
@@ -861,6 +898,13 @@ class SolveServer:
         self._retire(block=True)
 
 
+def _unlink_quiet(path: str):
+    try:
+        os.unlink(path)
+    except OSError:
+        pass
+
+
 def _spill_prefix() -> str:
     """A fresh path prefix for request files (memory-backed /dev/shm when present)."""
     base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
@@ -896,7 +940,7 @@ class NativeSolveServer:
                  free_fraction: float = 0.6, max_inflight: int = 256, max_desc: int = 1 << 16,
                  max_tasks: int = 1 << 20, fetch: str = "first", lower_procs: int = 4, lower_threads: int = 2,
                  tile_table: dict | None = None, lanes: int = 1, queue_depth: int = 2, placers: int = 2,
-                 result_pinned_bytes: int = 64 << 20, order: str = "longest"):
+                 result_pinned_bytes: int = 64 << 20, order: str = "longest", sim_threads: int = 4):
         if order not in ("longest", "admission"):
             raise ValueError(f"order must be 'longest' or 'admission', not {order!r}")
         if fetch not in ("first", "all", "none"):
@@ -942,18 +986,34 @@ class NativeSolveServer:
                 w.start()
         self._collector = threading.Thread(target=self._collect_main, daemon=True)
         self._collector.start()
+        # forward simulations of held requests: `sim_threads` threads with a
+        # stream each, so the simulations (latency-bound chains of small
+        # period launches) overlap on the device
+        self._simq: queue.Queue = queue.Queue()
+        self._tlock = threading.Lock()
+        self._simulators = [threading.Thread(target=self._simulate_main, daemon=True)
+                            for _ in range(max(1, int(sim_threads)))]
+        for t in self._simulators:
+            t.start()
 
     # -- client side --------------------------------------------------------------
-    def submit(self, model, params, t_submit: float | None = None) -> Request:
+    def submit(self, model, params, t_submit: float | None = None, simulate: dict | None = None) -> Request:
+        """Queue a solve. `simulate` = {"n_agents", "seed",
+        "initial_conditions"} also forward-simulates that many agents on the
+        solved value functions while they are still on the device (the
+        initial conditions: a dict of arrays, or `f(model, params, grids,
+        n_agents, rng) -> dict`); the request completes after both, with the
+        panel in `Request.simulation`."""
         req = Request(self._next_id, model, params, t_submit if t_submit is not None else time.perf_counter())
+        req.simulate = simulate
         self._next_id += 1
         with self._lock:
             self._by_rid[req.rid] = req
         if self._pool is not None:
             import cloudpickle
 
-            self._pool.submit(req.rid, self.dt.str, nat.REQ_TILE, cloudpickle.dumps((model, params)),
-                              self.fetch_mode)
+            blob = (model, params) if simulate is None else (model, params, simulate)
+            self._pool.submit(req.rid, self.dt.str, nat.REQ_TILE, cloudpickle.dumps(blob), self.fetch_mode)
         else:
             self._lowq.put(req)
         return req
@@ -986,7 +1046,8 @@ class NativeSolveServer:
             if req is None:
                 return
             try:
-                out = _lower_to_request_file((req.model, req.params), self.dt.str, self._spill, self.fetch_mode)
+                blob = (req.model, req.params) if req.simulate is None else (req.model, req.params, req.simulate)
+                out = _lower_to_request_file(blob, self.dt.str, self._spill, self.fetch_mode)
             except BaseException as e:  # surfaced to the client
                 out = e
             self._on_lowered(req.rid, out, -1)
@@ -1001,6 +1062,7 @@ class NativeSolveServer:
         if not isinstance(out, BaseException):
             req.worker_s = out["worker_s"]
             req.fetched = out["fetched"]
+            req.sim_path = out.get("sim_path")
             if self._L.lcm_serve_submit(self._h, rid, out["path"].encode()) == 0:
                 return
             out = nat.NativeKernelError(f"lcm_serve_submit failed: {nat.host_error()}")
@@ -1046,14 +1108,79 @@ class NativeSolveServer:
                             res.setdefault(t, {})[r] = buf[o: o + nb].view(self.dt).reshape(shape)
                             o += nb
                     req.result = res
+                    if req.sim_path is not None:  # held: the simulation thread finishes it
+                        self._simq.put(req)
+                        continue
                 else:
                     req.error = nat.NativeKernelError(err.value.decode(errors="replace"), status=rc)
+                    if req.sim_path is not None:
+                        _unlink_quiet(req.sim_path)
                 req.t_done = time.perf_counter()
                 req.done.set()
                 done.append(req)
             with self._lock:
                 self.completed.extend(done)
             self.timing["collect"] += time.perf_counter() - t0
+
+    # -- forward simulation -------------------------------------------------------
+    def _simulate_main(self):
+        """Place each held request's simulation image on this thread's stream,
+        simulate its agents on the request's resident value functions, then
+        release the request's device memory. The thread's panel and table
+        buffers are reused across requests (they only grow), so no free
+        synchronizes the device while the lanes run."""
+        import pickle
+        import types
+
+        from pylcm_amd._simulate import SimProgram
+
+        ctx = nat.Context()
+        bufs = [None, None]
+
+        def mem(slot, nbytes):
+            b = bufs[slot]
+            if b is None or b.nbytes < nbytes:
+                bufs[slot] = b = nat.DeviceBuffer(max(int(nbytes), 2 * (b.nbytes if b is not None else 0)))
+            return b.ptr
+
+        base, descs = C.c_uint64(), C.c_uint64()
+        while True:
+            req = self._simq.get()
+            if req is None:
+                return
+            held = False
+            try:
+                t0 = time.perf_counter()
+                with open(req.sim_path, "rb") as f:
+                    payload = pickle.load(f)
+                _unlink_quiet(req.sim_path)
+                nat.call("lcm_serve_held", self._h, req.rid, C.byref(base), C.byref(descs))
+                held = True
+                view = types.SimpleNamespace(ctx=ctx, base=0, desc_off=descs.value, dt=self.dt)
+                sim = SimProgram.for_placement(req.model, req.params, view)
+                sim.place(payload["img"], payload["ic"], payload["seed"], mem=mem)
+                t1 = time.perf_counter()
+                sim.launch()
+                t2 = time.perf_counter()
+                ctx.sync()
+                t3 = time.perf_counter()
+                req.simulation = sim.collect_raw()
+                t4 = time.perf_counter()
+                with self._tlock:
+                    tm = self.timing
+                    tm["sim_place"] = tm.get("sim_place", 0.0) + t1 - t0
+                    tm["sim_launch"] = tm.get("sim_launch", 0.0) + t2 - t1
+                    tm["sim_wait"] = tm.get("sim_wait", 0.0) + t3 - t2
+                    tm["sim_collect"] = tm.get("sim_collect", 0.0) + t4 - t3
+            except BaseException as e:  # surfaced to the client
+                req.error = e
+            finally:
+                if held:
+                    self._L.lcm_serve_release(self._h, req.rid)
+            req.t_simulated = req.t_done = time.perf_counter()
+            req.done.set()
+            with self._lock:
+                self.completed.append(req)
 
     # -- statistics ---------------------------------------------------------------
     def stats(self) -> dict:
@@ -1084,7 +1211,8 @@ class NativeSolveServer:
         its placers spent placing, and this process's collector spent
         collecting."""
         s = self.stats()
-        return {"launch": s["loop_ns"] / 1e9, "place": s["place_ns"] / 1e9, "collect": self.timing["collect"]}
+        return {"launch": s["loop_ns"] / 1e9, "place": s["place_ns"] / 1e9,
+                **{k: v for k, v in self.timing.items()}}
 
     def gpu_busy_seconds(self) -> float:
         """Time the device had at least one fused launch running (task upload
@@ -1106,6 +1234,12 @@ class NativeSolveServer:
         self._collector.join(timeout=60)
         if self._collector.is_alive():  # never free the server under a waiting collector
             raise RuntimeError("NativeSolveServer.close: the collector thread did not stop")
+        for _ in self._simulators:
+            self._simq.put(None)
+        for t in self._simulators:
+            t.join(timeout=60)
+            if t.is_alive():
+                raise RuntimeError("NativeSolveServer.close: a simulation thread did not stop")
         self._final = {"stats": self.stats(), "busy_s": self.gpu_busy_seconds()}
         self._L.lcm_serve_destroy(self._h)
         self._h = None
@@ -1121,7 +1255,8 @@ class NativeSolveServer:
 def serve_poisson(server: SolveServer, make_request, rate: float, duration: float, seed: int = 0):
     """Submit requests with exponential inter-arrival times (rate per second)
     for `duration` seconds while running the serving loop; then drain.
-    Returns the completed requests."""
+    `make_request(i)` returns (model, params) or, for the native server,
+    (model, params, simulate spec). Returns the completed requests."""
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     next_t = t0 + rng.exponential(1.0 / rate)
@@ -1129,8 +1264,11 @@ def serve_poisson(server: SolveServer, make_request, rate: float, duration: floa
     while True:
         now = time.perf_counter()
         while next_t <= now and next_t - t0 < duration:
-            model, params = make_request(len(submitted))
-            submitted.append(server.submit(model, params, t_submit=next_t))
+            item = make_request(len(submitted))
+            if len(item) > 2 and item[2] is not None:  # (model, params, simulate spec)
+                submitted.append(server.submit(item[0], item[1], t_submit=next_t, simulate=item[2]))
+            else:
+                submitted.append(server.submit(item[0], item[1], t_submit=next_t))
             next_t += rng.exponential(1.0 / rate)
         if next_t - t0 >= duration and now - t0 >= duration:
             break

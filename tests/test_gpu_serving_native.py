@@ -162,3 +162,51 @@ def test_native_request_that_fails_lowering_fails_alone(native):
         server.close()
     assert bad.error is not None and "LDS" in str(bad.error)
     _check(handles, reqs, periods="first", dtype=np.float64)
+
+
+def _plain_simulation(m, p, spec):
+    """The plain path: solve alone, then SimProgram with the same initial
+    conditions and seed."""
+    from pylcm_amd._simulate import SimProgram, _initial_conditions
+
+    prog = GsProgram(m, [p], np.float32)
+    prog.run(use_graph=False)
+    n, seed = spec["n_agents"], spec["seed"]
+    ic = _initial_conditions(m, spec["initial_conditions"](m, p, prog._rg[0], n, np.random.default_rng(seed)))
+    sim = SimProgram(m, p, prog)
+    sim.prepare(ic, seed)
+    sim.launch()
+    prog.ctx.sync()
+    return sim.collect_raw()
+
+
+@pytest.mark.parametrize("procs", [0, 2])
+def test_native_forward_simulation_per_request(native, procs):
+    """Requests with a forward simulation are held on the device after their
+    solve, simulated on their resident value functions and released: every
+    panel equals the plain solve-then-simulate path bit for bit, requests
+    without one are unaffected, and no device memory stays held."""
+    from pylcm_amd.serving import NativeSolveServer
+
+    reqs = _requests(8, seed=21, max_grid=300)
+    specs = [{"n_agents": 2000 + 500 * i, "seed": 100 + i, "initial_conditions": ex.serving_initial_conditions}
+             if i % 3 != 2 else None for i in range(len(reqs))]
+    server = NativeSolveServer(tile=0, fetch="all", budget_bytes=1 << 30, lower_procs=procs, lanes=2)
+    try:
+        handles = [server.submit(m, p, simulate=sp) for (m, p), sp in zip(reqs, specs)]
+        server.drain(180)
+        stats = server.stats()
+    finally:
+        server.close()
+    assert stats["bytes_used"] == 0 and stats["retired"] == len(reqs)
+    _check(handles, reqs)
+    for h, (m, p), sp in zip(handles, reqs, specs):
+        if sp is None:
+            assert h.simulation is None
+            continue
+        ref = _plain_simulation(m, p, sp)
+        assert sorted(h.simulation) == sorted(ref)
+        for k in ref:
+            assert h.simulation[k].shape == (m.n_periods, sp["n_agents"])
+            assert np.array_equal(h.simulation[k], ref[k], equal_nan=True), (h.rid, k)
+        assert h.t_retired <= h.t_simulated

@@ -172,3 +172,50 @@ def test_native_worker_writes_a_request_file(tmp_path):
     assert os.path.exists(out["path"]) and out["worker_s"] > 0
     rc, info = _inspect(out["path"])
     assert rc == 0 and info[3] == len(out["fetched"])
+
+
+def test_simulation_image_lists_every_pointer(tmp_path):
+    """The simulation image built where the solve is lowered: every pointer
+    word is listed once under its base, lies inside what that base covers,
+    and the EV descriptors point into the solve's descriptor array."""
+    from pylcm_amd._gs_program import DESC_DTYPE, GsProgram
+    from pylcm_amd._simulate import SimProgram
+
+    m, p = ex.stochastic_workload()
+    g = GsProgram.of([(m, p)], np.float32, tile=256, defer=True)
+    img = SimProgram(m, p, g).build_image()
+    words = img["image"].view(np.uint64)
+    seen = set()
+    limits = {SimProgram.BASE_TABLES: img["image"].nbytes, SimProgram.BASE_EV: img["ev_bytes"],
+              SimProgram.BASE_SOLVE: g.n_desc * DESC_DTYPE.itemsize}
+    for kind, offs in img["patch"].items():
+        assert np.all(offs % 8 == 0)
+        assert seen.isdisjoint(offs.tolist())
+        seen |= set(offs.tolist())
+        assert np.all(words[offs // 8] < np.uint64(limits[kind])), kind
+    n_ev = len(img["ev_keys"])
+    assert len(img["patch"][SimProgram.BASE_SOLVE]) == n_ev
+    assert np.all(words[img["patch"][SimProgram.BASE_SOLVE] // 8] % np.uint64(DESC_DTYPE.itemsize) == 0)
+    assert len(img["patch"][SimProgram.BASE_EV]) >= 3 * n_ev
+
+
+def test_worker_writes_a_held_request_with_its_simulation(tmp_path):
+    import pickle
+
+    from pylcm_amd.serving import _lower_to_request_file
+
+    m = ex.retirement_model(5, n_wealth=120, n_consumption=90, income_nodes=5)
+    p = ex.retirement_params(5, income={"rho": 0.9, "sigma": 0.1, "mu": 0.0})
+    spec = {"n_agents": 300, "seed": 4, "initial_conditions": ex.serving_initial_conditions}
+    out = _lower_to_request_file((m, p, spec), np.dtype(np.float32).str, str(tmp_path / "r_"), "first")
+    header = np.frombuffer(open(out["path"], "rb").read(128), dtype=np.int64)
+    assert header[10] == nat.REQ_HOLD
+    rc, info = _inspect(out["path"])
+    assert rc == 0, nat.host_error()
+    with open(out["sim_path"], "rb") as f:
+        payload = pickle.load(f)
+    assert payload["seed"] == 4 and payload["ic"]["_n"] == 300
+    assert set(payload["img"]["patch"]) == {0, 1, 2}
+    plain = _lower_to_request_file((m, p), np.dtype(np.float32).str, str(tmp_path / "q_"), "first")
+    assert "sim_path" not in plain
+    assert np.frombuffer(open(plain["path"], "rb").read(128), dtype=np.int64)[10] == 0
