@@ -817,7 +817,10 @@ def main():
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic,
             "traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) if traffic else None,
-            "kernel": f"gs_period_kernel<float, {prog.tile}, {max(prog.wdep_by_step)}>",
+            # the build the library picks for these launches (8 waves per SIMD
+            # when the launch is large and its LDS leaves room for them)
+            "kernel": f"gs_period_kernel<float, {prog.tile}, {max(prog.wdep_by_step)}, "
+                      f"{str(max(prog.n_tasks) >= nat.GS_DENSE_MIN_TASKS and (160 * 1024) // max(prog.lds_by_step) > 6).lower()}>",
             "avg_launch_ms": round(avg_ms, 4),
             "launches": len(per),
             "gflop_per_launch": round(flops_per_launch / 1e9, 3),
