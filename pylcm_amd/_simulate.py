@@ -131,6 +131,19 @@ class SimLowering(PeriodLowering):
         return np.ascontiguousarray(a.reshape(sl.n_sdd, sl.lay.n_d, sl.lay.n_c))
 
     def lower_sim(self, sl: SimLayout, period: int) -> SimTables:
+        """The simulation tables of (regime, period). Periods with the same
+        `period_key` (every function the tables read is age-free, and the
+        regime transition's ids and the active next regimes agree) share one
+        lowering, as the solve's tables do."""
+        key = self.period_key(sl.lay, period)
+        if key is None:
+            return self._lower_sim(sl, period)
+        memo_key = ("sim", sl.lay.name, key)
+        if memo_key not in self._memo:
+            self._memo[memo_key] = self._lower_sim(sl, period)
+        return self._memo[memo_key]
+
+    def _lower_sim(self, sl: SimLayout, period: int) -> SimTables:
         lay = sl.lay
         reg = lay.regime
         n_sdax = len(sl.sdd_idx)

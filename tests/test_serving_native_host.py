@@ -219,3 +219,19 @@ def test_worker_writes_a_held_request_with_its_simulation(tmp_path):
     plain = _lower_to_request_file((m, p), np.dtype(np.float32).str, str(tmp_path / "q_"), "first")
     assert "sim_path" not in plain
     assert np.frombuffer(open(plain["path"], "rb").read(128), dtype=np.int64)[10] == 0
+
+
+def test_simulation_image_same_with_period_sharing(monkeypatch):
+    """Periods with equal keys share one simulation lowering; the image and
+    its pointer lists are byte-identical to lowering every period."""
+    from pylcm_amd._gs_program import GsProgram
+    from pylcm_amd._simulate import SimLowering, SimProgram
+
+    m, p = ex.stochastic_workload()
+    g = GsProgram.of([(m, p)], np.float32, tile=256, defer=True)
+    shared = SimProgram(m, p, g).build_image()
+    monkeypatch.setattr(SimLowering, "period_key", lambda self, lay, period: None)
+    every = SimProgram(m, p, g).build_image()
+    assert shared["image"].tobytes() == every["image"].tobytes()
+    assert all(np.array_equal(shared["patch"][k], every["patch"][k]) for k in every["patch"])
+    assert shared["lds_max"] == every["lds_max"] and shared["ev_keys"] == every["ev_keys"]
