@@ -121,15 +121,25 @@ def dcegm_leg(args):
     for _ in range(reps):
         prog.run(sync=False)
     prog.ctx.sync()
+    ms_solve_eager = 1e3 * (time.perf_counter() - t0) / reps
+    V_eager = prog.value_functions()
+    # the same launches replayed as one HIP graph (the timed figure)
+    prog.run(use_graph=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prog.run(sync=False, use_graph=True)
+    prog.ctx.sync()
     ms_solve = 1e3 * (time.perf_counter() - t0) / reps
     e0, e1 = nat.Event(), nat.Event()
     e0.record(prog.ctx)
-    prog.run(sync=False)
+    prog.run(sync=False, use_graph=True)
     e1.record(prog.ctx)
     kernel_ms = e0.elapsed_ms(e1)
     V = prog.value_functions()
+    graph_same = all(np.array_equal(V[t][r], V_eager[t][r]) for t in V for r in V[t])
     n_cols = sum(v[1] for v in prog._period.values())
-    log(f"dcegm: {ms_solve:.2f} ms per solve ({kernel_ms:.2f} ms on the stream), {len(prog._period)} periods, "
+    log(f"dcegm: {ms_solve:.2f} ms per solve as a graph ({ms_solve_eager:.2f} eager, {kernel_ms:.2f} ms on the stream; "
+        f"graph equals eager: {graph_same}), {len(prog._period)} periods, "
         f"{n_cols} columns")
     # the grid-search answer on a 4000-point consumption grid
     g = GsProgram(ex.dcegm_retirement_model(T, grid_search=True), [p], dt)
@@ -149,6 +159,8 @@ def dcegm_leg(args):
                     "taste shocks 0.05, FUES on every column",
         "ms_per_solve": round(ms_solve, 3),
         "solves_per_s": round(1e3 / ms_solve, 2),
+        "ms_per_solve_eager": round(ms_solve_eager, 3),
+        "graph_equals_eager": bool(graph_same),
         "stream_ms_per_solve": round(kernel_ms, 3),
         "columns": n_cols,
         "host_build_s": round(build_s, 3),
@@ -167,10 +179,11 @@ def dcegm_leg(args):
         batch = EgmBatch(m, plist, dt, ctx=prog.ctx)
         bbuild = time.perf_counter() - t0
         batch.run()
+        batch.run(use_graph=True)
         breps = 3
         t0 = time.perf_counter()
         for _ in range(breps):
-            batch.run(sync=False)
+            batch.run(sync=False, use_graph=True)
         prog.ctx.sync()
         ms_batch = 1e3 * (time.perf_counter() - t0) / breps
         Vb = batch.value_functions(0)

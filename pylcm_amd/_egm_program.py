@@ -82,6 +82,7 @@ class EgmProgram:
         if device:
             _abi_check()
         self.nodes_per_block = int(nodes_per_block)  # 0: chosen per launch by the library
+        self._graph = None  # the period loop captured as a HIP graph (`run(use_graph=True)`)
         self.model = model
         self.params = params
         self.dt = np.dtype(dtype)
@@ -413,13 +414,28 @@ class EgmProgram:
     def _pack(self):
         self._reserve()
         self.buf = nat.DeviceBuffer(max(self.ar.size, 256))
+        self._graph = None  # a captured graph addresses the previous buffer
         self.base = self.buf.ptr
         img, _, self.lds_by_t = self._image(self.base)
         nat.h2d(self.ctx, self.base, img[: self.ar.size])
         self.ctx.sync()
 
     # ------------------------------------------------------------------ running
-    def run(self, sync=True):
+    def run(self, sync=True, use_graph=False):
+        """Every period's launches on the program's stream, newest period
+        first. `use_graph`: replay them as one HIP graph (captured on first
+        use; kernel nodes only), which saves the host one foreign call per
+        launch; the arithmetic is the same launches'."""
+        if use_graph:
+            if self._graph is None:
+                self._graph = nat.Graph.capture(self.ctx, self._enqueue)
+            self._graph.launch(self.ctx)
+        else:
+            self._enqueue()
+        if sync:
+            self.ctx.sync()
+
+    def _enqueue(self):
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         base = self.base
@@ -430,8 +446,6 @@ class EgmProgram:
                                            max_rows, max_s, self.nodes_per_block), f"lcm_egm_launch (period {t})")
             nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
                        f"lcm_egm_aggregate (period {t})")
-        if sync:
-            self.ctx.sync()
 
     def launch_regime(self, t: int, r: str):
         """The columns and aggregates of one (period, regime) alone: the
@@ -522,6 +536,7 @@ class EgmBatch:
     def __init__(self, model, params_list, dtype, ctx=None, nodes_per_block: int = 0):
         _abi_check()
         self.nodes_per_block = int(nodes_per_block)  # 0: chosen per launch by the library
+        self._graph = None  # the period loop captured as a HIP graph (`run(use_graph=True)`)
         self.dt = np.dtype(dtype)
         self.ctx = ctx or nat.Context()
         self.progs = [EgmProgram(model, p, dtype, device=False) for p in params_list]
@@ -543,6 +558,7 @@ class EgmBatch:
             self._fused[t] = (col_off, n_cols, agg_off, n_aggs)
         self.total_bytes = total
         self.buf = nat.DeviceBuffer(max(total, 256))
+        self._graph = None  # a captured graph addresses the previous buffer
         base = self.buf.ptr
         cols = {t: [] for t in periods}
         aggs = {t: [] for t in periods}
@@ -569,7 +585,19 @@ class EgmBatch:
                 nat.h2d(self.ctx, base + agg_off, a)
         self.ctx.sync()
 
-    def run(self, sync=True):
+    def run(self, sync=True, use_graph=False):
+        """The batch's fused launches, newest period first; `use_graph` as
+        `EgmProgram.run`."""
+        if use_graph:
+            if self._graph is None:
+                self._graph = nat.Graph.capture(self.ctx, self._enqueue)
+            self._graph.launch(self.ctx)
+        else:
+            self._enqueue()
+        if sync:
+            self.ctx.sync()
+
+    def _enqueue(self):
         L = nat.lib()
         code = nat.LCM_DTYPE_F64 if self.dt == np.float64 else nat.LCM_DTYPE_F32
         base = self.buf.ptr
@@ -580,8 +608,6 @@ class EgmBatch:
                                            max_rows, max_s, self.nodes_per_block), f"lcm_egm_launch (period {t})")
             nat._check(L.lcm_egm_aggregate(self.ctx.handle, code, C.c_void_p(base + agg_off), n_aggs, max_w),
                        f"lcm_egm_aggregate (period {t})")
-        if sync:
-            self.ctx.sync()
 
     def value_functions(self, i):
         return self.progs[i].value_functions()

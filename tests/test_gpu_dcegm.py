@@ -144,3 +144,36 @@ def test_dcegm_batch_matches_single_solves(native, dtype, nodes):
     # the model API routes batched DC-EGM solves here
     Vapi = m.solve_batch(plist[:2], dtype=dtype)
     assert all(np.array_equal(Vapi[1][t][r], batch.value_functions(1)[t][r]) for t in Vapi[1] for r in Vapi[1][t])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_dcegm_graph_replays_equal_eager(native, dtype):
+    """The DC-EGM solve replayed as one HIP graph gives the eager solve's V
+    and carry rows bit for bit: a program that only ever runs the graph, and
+    replays interleaved with the same program's eager runs; likewise for a
+    batch."""
+    from pylcm_amd._egm_program import EgmBatch, EgmProgram
+
+    m = ex.dcegm_retirement_model(5, n_savings=200, savings_max=100.0, n_wealth=90, wealth_range=(0.05, 100.0))
+    plist = [ex.dcegm_params(5, discount_factor=b, crra=c, sigma=s)
+             for b, c, s in [(0.96, 2.0, 0.15), (0.93, 1.5, 0.1)]]
+    eager = EgmProgram(m, plist[0], dtype)
+    eager.run()
+    V0 = eager.value_functions()
+    C0 = {k: eager.columns(*k) for k in eager.col_out}
+    graph = EgmProgram(m, plist[0], dtype)
+    for k in range(3):
+        graph.run(use_graph=True)
+        V, Cols = graph.value_functions(), {kk: graph.columns(*kk) for kk in graph.col_out}
+        assert all(np.array_equal(V[t][r], V0[t][r]) for t in V0 for r in V0[t]), k
+        assert all(np.array_equal(Cols[kk][f], C0[kk][f], equal_nan=True) for kk in C0 for f in C0[kk]), k
+        graph.run()  # the program's own eager work between replays
+    be = EgmBatch(m, plist, dtype)
+    be.run()
+    bg = EgmBatch(m, plist, dtype)
+    for k in range(2):
+        bg.run(use_graph=True)
+        for i in range(len(plist)):
+            Vg, Ve = bg.value_functions(i), be.value_functions(i)
+            assert all(np.array_equal(Vg[t][r], Ve[t][r]) for t in Ve for r in Ve[t]), (k, i)
+        bg.run()
