@@ -28,6 +28,7 @@
 // the loop. Blocks are remapped so that consecutive tasks (same problem, same
 // V rows) land on the same XCD and share its L2.
 
+#include <algorithm>
 #include <type_traits>
 
 #include "common.h"
@@ -613,21 +614,25 @@ gs_period_kernel(const LcmGsDesc* __restrict__ descs,
   const int g_n = interp ? D->g_n : 0;
   const int n_constr = D->n_constr;
 
-  // LDS carve (the host sizes the launch for the largest g_n and n_constr)
+  // LDS carve (the host sizes the launch for the largest g_n and n_constr).
+  // The first region holds the EV row (and the irregular grid points) while
+  // the actions are scanned, and the SPLIT wave groups' partial results when
+  // they are merged: every merge starts with a barrier after the last read of
+  // the row, and the next row is written after the next barrier
   const size_t g_bytes = align16(size_t(g_n > 0 ? g_n : 1) * sizeof(T));
+  const size_t row_bytes = max(2 * g_bytes, size_t(kBlock) * (sizeof(T) + 8));
   // EV row: contiguous, or in the fast variant the .v members of the pairs
   constexpr int ES = kEvStride<VAR>;
   T* ev = reinterpret_cast<T*>(smem);
   Pair<T>* pairs = reinterpret_cast<Pair<T>*>(smem);
   T* gp = reinterpret_cast<T*>(smem + g_bytes);  // irregular grid points (not the fast variant)
-  Act<T>* act = reinterpret_cast<Act<T>*>(smem + 2 * g_bytes);
-  T* his = reinterpret_cast<T*>(smem + 2 * g_bytes + CH * sizeof(Act<T>));
-  const size_t con_ofs = 2 * g_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
+  Act<T>* act = reinterpret_cast<Act<T>*>(smem + row_bytes);
+  T* his = reinterpret_cast<T*>(smem + row_bytes + CH * sizeof(Act<T>));
+  const size_t con_ofs = row_bytes + CH * sizeof(Act<T>) + align16(CH * sizeof(T));
   Con<T>* con = reinterpret_cast<Con<T>*>(smem + con_ofs);  // constraint rows (staged variant)
   T* pm = reinterpret_cast<T*>(smem + con_ofs);              // fast variant: prefix max of lo
   T* wtot = reinterpret_cast<T*>(smem + con_ofs + align16(CH * sizeof(T)));  // its per-wave maxima
-  const size_t pm_bytes = FAST ? align16(CH * sizeof(T)) + 64 : 0;
-  T* red_v = reinterpret_cast<T*>(smem + con_ofs + pm_bytes + size_t(FAST ? 0 : n_constr) * CH * sizeof(Con<T>));
+  T* red_v = reinterpret_cast<T*>(smem);  // shares the row region (see above)
   int* red_i = reinterpret_cast<int*>(red_v + kBlock);
   int* red_f = red_i + kBlock;
 
@@ -919,8 +924,8 @@ extern "C" size_t lcm_gs_lds_bytes(int dtype, int max_g, int max_constr) {
   const size_t sz = dtype == LCM_DTYPE_F64 ? 8 : 4;
   const size_t g = (size_t(max_g > 0 ? max_g : 1) * sz + 15) & ~size_t(15);
   const size_t his = (lcm::kActChunk * sz + 15) & ~size_t(15);
-  return 2 * g + lcm::kActChunk * 4 * sz + 2 * his + 64 + size_t(max_constr) * lcm::kActChunk * 2 * sz +
-         lcm::kBlock * (sz + 8) + 64;
+  const size_t row = std::max(2 * g, size_t(lcm::kBlock) * (sz + 8));  // EV row, or the merge's partials
+  return row + lcm::kActChunk * 4 * sz + 2 * his + 64 + size_t(max_constr) * lcm::kActChunk * 2 * sz + 64;
 }
 
 extern "C" int lcm_gs_launch_ex(lcm_ctx* ctx, int dtype, const LcmGsDesc* d_descs,
