@@ -18,6 +18,11 @@ are the same (tests check this against `GsProgram`).
 from __future__ import annotations
 
 import ctypes as C
+import glob
+import itertools
+import os
+import tempfile
+import uuid
 
 import numpy as np
 
@@ -83,14 +88,64 @@ def _init_worker(models_blob: bytes):
     _MODELS.update(cloudpickle.loads(models_blob))
 
 
-def _pack_chunk(items, dtype: str, tile: int) -> list:
-    """Pack [(model key, params), ...] with the worker's models."""
-    return [pack_problem(_MODELS[k], p, dtype, tile) for k, p in items]
+_SPILL_IDS = itertools.count()
+
+
+def spill_prefix() -> str:
+    """A fresh path prefix for files handed between processes (memory-backed
+    /dev/shm when present)."""
+    base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
+    return os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
+
+
+def unlink_quiet(path: str):
+    try:
+        os.unlink(path)
+    except OSError:
+        pass
+
+
+def _pack_chunk(items, dtype: str, tile: int, spill: str | None = None) -> list:
+    """Pack [(model key, params), ...] with the worker's models. With `spill`
+    the table images go into one file at that prefix (`spilled` = (path,
+    offset, size) per problem) instead of through the result pipe: the
+    images are most of a packed problem's bytes, and a file in memory is
+    read by the parent without a copy."""
+    packs = [pack_problem(_MODELS[k], p, dtype, tile) for k, p in items]
+    if spill is None or not packs:
+        return packs
+    path = f"{spill}{os.getpid()}_{next(_SPILL_IDS)}"
+    off = 0
+    with open(path, "wb") as f:
+        for p in packs:
+            img = np.ascontiguousarray(p.pop("image")).view(np.uint8).reshape(-1)
+            f.write(memoryview(img))
+            p["spilled"] = (path, off, int(img.size))
+            off += int(img.size)
+    return packs
+
+
+def _unspill_chunk(packs: list):
+    """Map the file `_pack_chunk` spilled the chunk's images into and point
+    every problem's `image` at its bytes; the file is removed at once (the
+    mapping keeps its pages until the images are dropped)."""
+    if not packs or "spilled" not in packs[0]:
+        return
+    path = packs[0]["spilled"][0]
+    try:
+        total = sum(p["spilled"][2] for p in packs)
+        mm = np.memmap(path, dtype=np.uint8, mode="r", shape=(total,)) if total else np.zeros(0, np.uint8)
+    finally:
+        unlink_quiet(path)
+    for p in packs:
+        _, off, n = p.pop("spilled")
+        p["image"] = mm[off: off + n]
 
 
 class WorkerPool:
     """Spawned worker processes holding `models` (sent once, by cloudpickle);
-    `submit(draws)` packs [(model key, params), ...] in chunks."""
+    `submit(draws)` packs [(model key, params), ...] in chunks. The table
+    images come back through memory-backed files (see `_pack_chunk`)."""
 
     def __init__(self, models: dict, procs: int, dtype, tile: int, chunk: int = 16):
         import multiprocessing
@@ -101,6 +156,7 @@ class WorkerPool:
         self.dt = np.dtype(dtype)
         self.tile = int(tile)
         self.chunk = int(chunk)
+        self.spill = spill_prefix()
         self.pool = ProcessPoolExecutor(max_workers=int(procs), mp_context=multiprocessing.get_context("spawn"),
                                         initializer=_init_worker, initargs=(cloudpickle.dumps(models),))
         # start every worker now, so the first batch pays no start-up
@@ -110,18 +166,23 @@ class WorkerPool:
     def submit(self, draws):
         """Futures of packed chunks, in the order of `draws`."""
         draws = list(draws)
-        return [self.pool.submit(_pack_chunk, draws[i: i + self.chunk], self.dt.str, self.tile)
+        return [self.pool.submit(_pack_chunk, draws[i: i + self.chunk], self.dt.str, self.tile, self.spill)
                 for i in range(0, len(draws), self.chunk)]
 
     @staticmethod
     def gather(futures) -> list:
         out = []
         for f in futures:
-            out.extend(f.result())
+            chunk = f.result()
+            _unspill_chunk(chunk)
+            out.extend(chunk)
         return out
 
     def close(self):
         self.pool.shutdown(wait=True, cancel_futures=True)
+        # files of chunks that were packed but never gathered
+        for path in glob.glob(glob.escape(self.spill) + "*"):
+            unlink_quiet(path)
 
 
 # -- assembly on the device ---------------------------------------------------------
@@ -377,9 +438,11 @@ class FreshBatches:
             return stats
 
         def lower_and_place(k, pending):
+            # batch k + 1 is queued before batch k is gathered, so workers that
+            # finish their last chunk of batch k go on with the next batch
+            nxt = self.pool.submit(batches[k + 1]) if k + 1 < len(batches) else None
             a = time.perf_counter()
             packed = self.pool.gather(pending)
-            nxt = self.pool.submit(batches[k + 1]) if k + 1 < len(batches) else None
             b = time.perf_counter()
             ab = self._place(packed, k)
             return ab, nxt, {"wait_workers_s": b - a, "place_s": time.perf_counter() - b}

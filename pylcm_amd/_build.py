@@ -288,6 +288,7 @@ class PeriodLowering:
         self.rg = rg
         self._memo = {}
         self._age_free_memo = {}
+        self._period_key_memo = {}
 
     def _age_free(self, lay, funcs) -> bool:
         """Whether none of `funcs` reads `age` or `period`, directly or through
@@ -474,25 +475,33 @@ class PeriodLowering:
         other period with the same key, or None. Holds when every function the
         lowering evaluates is age-free except a deterministic regime transition
         that broadcasts over the age (its ids then enter the key)."""
+        kind = self._period_key_memo.get(lay.name)
+        if kind is None:
+            kind = self._period_key_memo[lay.name] = self._period_key_kind(lay)
+        if kind[0] == "terminal":
+            return ("terminal", lay.name)
+        if kind[0] is None or period == self.model.n_periods - 1:
+            return None
+        active_next = tuple(self.model.active_regimes(period + 1))
+        return ("period", lay.name, active_next, kind[1][period].tobytes())
+
+    def _period_key_kind(self, lay):
+        """The period-independent half of `period_key` for one regime:
+        ("terminal",), ("period", ids of the regime transition per period) or
+        (None,) when no two periods are known to share tables."""
         reg = lay.regime
-        T = self.model.n_periods
         funcs = [reg.functions["utility"]] + [f for f in reg.constraints.values() if f is not None]
         if lay.terminal:
-            return ("terminal", lay.name) if self._age_free(lay, funcs) else None
-        if period == T - 1:
-            return None
+            return ("terminal",) if self._age_free(lay, funcs) else (None,)
         tr = reg.transition
         if isinstance(tr, MarkovTransition) or hasattr(tr, "items"):
-            return None
+            return (None,)
         for f in reg.state_transitions.values():
             funcs += list(f.values()) if hasattr(f, "items") else [f]
         if not self._age_free(lay, funcs):
-            return None
+            return (None,)
         ids_all = self._ids_all_periods(lay, tr)
-        if ids_all is None:
-            return None
-        active_next = tuple(self.model.active_regimes(period + 1))
-        return ("period", lay.name, active_next, ids_all[period].tobytes())
+        return (None,) if ids_all is None else ("period", ids_all)
 
     def regime_probs(self, lay, period):
         """p[sd, d, regime_code] of moving to each regime."""

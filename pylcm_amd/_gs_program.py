@@ -423,14 +423,20 @@ class GsProgram:
         """`_describe`, reusing the work of an earlier period whose tables are
         the same objects (the lowering shares tables that do not depend on
         the age): only the output and next-period V offsets differ."""
-        key = (id(model), p, r, id(pt.U), id(pt.F), id(pt.A), id(pt.B), pt.terminal, pt.beta,
-               pt.taste_scale, pt.next_grid, tuple((n, id(M)) for n, M in pt.targets),
-               tuple((n, id(m)) for n, m in pt.scalars))
-        hit = cache.get(key)
+        # periods that share their tables share the tables object itself: a
+        # first lookup by its identity skips building the content key
+        idk = (id(model), p, r, id(pt))
+        hit = cache.get(idk)
         if hit is None:
-            d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
-            self._templates.append(d)
-            hit = cache[key] = (len(self._templates) - 1, pt)  # pt keeps the keyed tables alive
+            key = (id(model), p, r, id(pt.U), id(pt.F), id(pt.A), id(pt.B), pt.terminal, pt.beta,
+                   pt.taste_scale, pt.next_grid, tuple((n, id(M)) for n, M in pt.targets),
+                   tuple((n, id(m)) for n, m in pt.scalars))
+            hit = cache.get(key)
+            if hit is None:
+                d = self._describe(model, p, t, r, pt, rg, ar, zero_off)
+                self._templates.append(d)
+                hit = cache[key] = (len(self._templates) - 1, pt)  # pt keeps the keyed tables alive
+            cache[idk] = (hit[0], pt)  # this pt keeps its own identity key valid
         out = self.outs[p][(t, r)]
         vn = [-1] * nat.MAX_TARGETS
         vs = [-1] * nat.MAX_SCALAR

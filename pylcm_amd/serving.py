@@ -23,16 +23,15 @@ from __future__ import annotations
 import ctypes as C
 import os
 import queue
-import tempfile
 import threading
 import time
-import uuid
 from dataclasses import dataclass, field
 
 import numpy as np
 
 from pylcm_amd import _native as nat
 from pylcm_amd import autotune
+from pylcm_amd._batch import spill_prefix, unlink_quiet
 from pylcm_amd._gs_program import DESC_DTYPE, TASK_DTYPE, GsProgram, lower_problem, step_lds_bytes
 
 ALIGN = 256
@@ -524,8 +523,7 @@ class SolveServer:
         if lower_procs > 0:
             # worker results' bulky arrays travel through files of this prefix
             # (memory-backed /dev/shm when present); close() removes leftovers
-            base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
-            self._spill = os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
+            self._spill = spill_prefix()
             self._by_rid: dict[int, Request] = {}
             self._pool = _LoweringPool(int(lower_procs), self._spill, self._on_lowered)
             lower_threads = max(lower_threads, 2)
@@ -898,19 +896,6 @@ class SolveServer:
         self._retire(block=True)
 
 
-def _unlink_quiet(path: str):
-    try:
-        os.unlink(path)
-    except OSError:
-        pass
-
-
-def _spill_prefix() -> str:
-    """A fresh path prefix for request files (memory-backed /dev/shm when present)."""
-    base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
-    return os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
-
-
 class NativeSolveServer:
     """Continuous batching of grid-search solves with the serving loop in the
     library (`lcm_serve_*`): its threads place requests in device memory,
@@ -966,7 +951,7 @@ class NativeSolveServer:
                  int(queue_depth), int(placers), int(tile), self._t_units.ctypes.data, self._t_acts.ctypes.data,
                  self._t_tiles.ctypes.data, len(ent), nat.SERVE_LONGEST_FIRST if order == "longest" else 0)
         self.budget = budget
-        self._spill = _spill_prefix()
+        self._spill = spill_prefix()
         self._lock = threading.Lock()
         self._by_rid: dict[int, Request] = {}
         self.completed: list[Request] = []
@@ -1114,7 +1099,7 @@ class NativeSolveServer:
                 else:
                     req.error = nat.NativeKernelError(err.value.decode(errors="replace"), status=rc)
                     if req.sim_path is not None:
-                        _unlink_quiet(req.sim_path)
+                        unlink_quiet(req.sim_path)
                 req.t_done = time.perf_counter()
                 req.done.set()
                 done.append(req)
@@ -1153,7 +1138,7 @@ class NativeSolveServer:
                 t0 = time.perf_counter()
                 with open(req.sim_path, "rb") as f:
                     payload = pickle.load(f)
-                _unlink_quiet(req.sim_path)
+                unlink_quiet(req.sim_path)
                 nat.call("lcm_serve_held", self._h, req.rid, C.byref(base), C.byref(descs))
                 held = True
                 view = types.SimpleNamespace(ctx=ctx, base=0, desc_off=descs.value, dt=self.dt)

@@ -267,3 +267,34 @@ def test_worker_result_spill_round_trip(tmp_path):
     assert len(spilled["tasks_host"]) == len(plain["tasks_host"])
     for a, b in zip(spilled["tasks_host"], plain["tasks_host"]):
         assert np.array_equal(a, b)
+
+
+def test_fresh_batch_workers_hand_images_over_in_files():
+    """The fresh-batch worker pool packs chunks in spawned processes and hands
+    the table images back through memory-backed files: every packed problem
+    equals packing it in this process, and no file is left behind, also for
+    chunks that were packed but never gathered."""
+    import glob
+
+    import numpy as np
+
+    from pylcm_amd import examples as ex
+    from pylcm_amd._batch import WorkerPool, pack_problem
+
+    T = 6
+    models = {n: ex.retirement_model(T, n_wealth=n, n_consumption=n, income_nodes=3) for n in (20, 40)}
+    draws = [(20 if i % 2 else 40, p) for i, (_, p) in enumerate(ex.draw_batch_params(5, n_periods=T, seed=4))]
+    pool = WorkerPool(models, 2, np.float32, 128, chunk=2)
+    try:
+        got = pool.gather(pool.submit(draws))
+        pool.submit(draws[:3])  # left for close() to clean up
+    finally:
+        pool.close()
+    assert glob.glob(glob.escape(pool.spill) + "*") == []
+    assert len(got) == len(draws)
+    for (k, p), g in zip(draws, got):
+        ref = pack_problem(models[k], p, np.float32, 128)
+        assert "spilled" not in g
+        assert np.asarray(g["image"]).tobytes() == ref["image"].tobytes()
+        assert g["D0"].tobytes() == ref["D0"].tobytes()
+        assert np.array_equal(g["ptr_cls"], ref["ptr_cls"]) and np.array_equal(g["tasks_cat"], ref["tasks_cat"])
