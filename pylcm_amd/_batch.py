@@ -263,12 +263,14 @@ class AssembledBatch:
         # gather: destination row d of block (s, i) reads source row d - (dst - src)
         rows = np.repeat(dT, cT) + (np.arange(int(cT.sum())) - np.repeat(np.cumsum(cT) - cT, cT))
         srows = rows - np.repeat(dT - sT, cT)
-        flat = np.zeros(total, dtype=TASK_DTYPE)
-        flat["desc"] = -1
-        dbase = np.repeat(np.broadcast_to(self.desc_base[:, None], (n, S)).T.ravel(), cT).astype(np.int32)
-        flat["desc"][rows] = allt[srows, 0] + dbase
-        flat["sd"][rows] = allt[srows, 1]
-        flat["w0"][rows] = allt[srows, 2]
+        # rows of four int32 (desc, sd, w0, pad), viewed as task records at the end
+        assert TASK_DTYPE.itemsize == 16 and TASK_DTYPE.names[:3] == ("desc", "sd", "w0")
+        flat4 = np.zeros((total, 4), dtype=np.int32)
+        flat4[:, 0] = -1
+        moved = allt[srows].astype(np.int32)
+        moved[:, 0] += np.repeat(np.broadcast_to(self.desc_base[:, None], (n, S)).T.ravel(), cT).astype(np.int32)
+        flat4[rows, :3] = moved
+        flat = flat4.view(TASK_DTYPE).reshape(-1)
         self.tasks = [flat[step_start[s]: step_start[s + 1]] for s in range(S)]
         self.task_off = []
         for s in range(S):
@@ -297,19 +299,29 @@ class AssembledBatch:
         (written into `into`, a uint8 array of n_desc records, when given): a
         pointer below the problem's `out_start` addresses its outputs, the
         others its tables."""
-        n = self.n_desc
-        raw = np.empty(n * DESC_DTYPE.itemsize, dtype=np.uint8) if into is None else into
-        np.concatenate([p["D0"].view(np.uint8).reshape(-1) for p in self.packed], out=raw)
-        counts = [p["n_desc"] for p in self.packed]
+        raw = np.empty(self.n_desc * DESC_DTYPE.itemsize, dtype=np.uint8) if into is None else into
+        self._records_of(0, len(self.packed), raw)
+        return raw.view(DESC_DTYPE)
+
+    def _records_of(self, lo: int, hi: int, raw: np.ndarray):
+        """`records` of problems [lo, hi) into `raw` (all records' bytes)."""
+        packed = self.packed[lo:hi]
+        if not packed:
+            return
+        sz = DESC_DTYPE.itemsize
+        d0, d1 = int(self.desc_base[lo]), int(self.desc_base[hi - 1]) + packed[-1]["n_desc"]
+        out = raw[d0 * sz: d1 * sz]
+        n = d1 - d0
+        np.concatenate([p["D0"].view(np.uint8).reshape(-1) for p in packed], out=out)
+        counts = [p["n_desc"] for p in packed]
         base = np.uint64(self.base)
         lut = np.zeros((n, 3), dtype=np.uint64)
-        lut[:, 1] = np.repeat(base + self.out_offs.astype(np.uint64), counts)
-        start = np.asarray([p["out_start"] for p in self.packed], dtype=np.uint64)
-        lut[:, 2] = np.repeat(base + self.tab_offs.astype(np.uint64) - start, counts)
-        cls = np.concatenate([p["ptr_cls"] for p in self.packed])
-        W = raw.view(np.uint64).reshape(n, WORDS)
+        lut[:, 1] = np.repeat(base + self.out_offs[lo:hi].astype(np.uint64), counts)
+        start = np.asarray([p["out_start"] for p in packed], dtype=np.uint64)
+        lut[:, 2] = np.repeat(base + self.tab_offs[lo:hi].astype(np.uint64) - start, counts)
+        cls = np.concatenate([p["ptr_cls"] for p in packed])
+        W = out.view(np.uint64).reshape(n, WORDS)
         W[:, :PTR_WORDS] += np.take_along_axis(lut, cls.astype(np.intp), axis=1)
-        return raw.view(DESC_DTYPE)
 
     def host_image(self, into=None) -> np.ndarray:
         """Bytes [tables_off, total_bytes) of the buffer: the tables,
@@ -327,13 +339,15 @@ class AssembledBatch:
         k = min(8, max(1, n // 16))
         bounds = np.linspace(0, n, k + 1).astype(int)
         if k > 1:
-            # large copies release the GIL: the tables go in parallel
+            # large copies and array operations release the GIL: the tables
+            # and the records of k problem ranges go in parallel
             from concurrent.futures import ThreadPoolExecutor
 
+            o = self.desc_off - t0
+            recs = img[o: o + self.n_desc * DESC_DTYPE.itemsize]
             with ThreadPoolExecutor(k) as ex:
                 jobs = [ex.submit(copy, int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:])]
-                o = self.desc_off - t0
-                self.records(into=img[o: o + self.n_desc * DESC_DTYPE.itemsize])
+                jobs += [ex.submit(self._records_of, int(a), int(b), recs) for a, b in zip(bounds[:-1], bounds[1:])]
                 for j in jobs:
                     j.result()
         else:
