@@ -91,10 +91,18 @@ def _init_worker(models_blob: bytes):
 _SPILL_IDS = itertools.count()
 
 
+SPILL_MIN_FREE = 4 << 30  # bytes /dev/shm must have free to be used for spill files
+
+
 def spill_prefix() -> str:
-    """A fresh path prefix for files handed between processes (memory-backed
-    /dev/shm when present)."""
-    base = "/dev/shm" if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK) else tempfile.gettempdir()
+    """A fresh path prefix for files handed between processes: memory-backed
+    /dev/shm when present, writable and not small (container defaults can be
+    64 MiB), else the temporary directory."""
+    base = tempfile.gettempdir()
+    if os.path.isdir("/dev/shm") and os.access("/dev/shm", os.W_OK):
+        st = os.statvfs("/dev/shm")
+        if st.f_bavail * st.f_frsize >= SPILL_MIN_FREE:
+            base = "/dev/shm"
     return os.path.join(base, f"pylcm_amd_{os.getpid()}_{uuid.uuid4().hex[:8]}_")
 
 
